@@ -1,0 +1,247 @@
+/*
+ * raymond-amd C ABI: the drop-in boundary between a render-loop host and the
+ * MI355X (gfx950) wavefront path tracer.
+ *
+ * The reference has no FFI layer; its Swift host drives Metal directly.  Each
+ * entry point below replaces one reference interface (cited per function):
+ *   scene build    host/scene/SceneLoader.swift:46-143 (+ Shape/Entity/Light builders)
+ *   material JIT   host/scene/MaterialBuilder.swift:253-325,889-903 (Metal JIT -> hiprtc)
+ *   render loop    host/Renderer.swift:480-696 (execute), :757-798 (setSize), :729-732 (reset)
+ *   readback       host/Renderer.swift:804-807 (saveFrame), normalizeImage.metal:39-59
+ *
+ * Conventions: opaque handles, int status (RM_OK = 0, negative on error),
+ * thread-local message in rm_last_error().  All arrays are caller-owned and
+ * copied; no torch or HIP types appear in signatures.  A renderer is
+ * single-threaded with at most one rm_render in flight (the reference's
+ * DispatchSemaphore(1), host/Renderer.swift:285,481-482).
+ */
+#ifndef RAYMOND_AMD_RM_H
+#define RAYMOND_AMD_RM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RM_OK 0
+#define RM_ERROR_INVALID (-1)
+#define RM_ERROR_HIP (-2)
+#define RM_ERROR_COMPILE (-3)
+#define RM_ERROR_STATE (-4)
+#define RM_ERROR_OOM (-5)
+
+typedef struct rm_device rm_device;
+typedef struct rm_scene rm_scene;
+typedef struct rm_renderer rm_renderer;
+
+/* Ray flags (bridge/Ray.hpp:6-15), used for instance visibility masks. */
+#define RM_RAY_CAMERA 1u
+#define RM_RAY_REFLECTION 2u
+#define RM_RAY_TRANSMISSION 4u
+#define RM_RAY_SHADOW 8u
+#define RM_RAY_VOLUME 16u
+#define RM_RAY_DIFFUSE 32u
+#define RM_RAY_GLOSSY 64u
+#define RM_RAY_SINGULAR 128u
+#define RM_RAY_ALL 255u
+
+/* Sampling modes (bridge/Uniforms.hpp:5-9) */
+#define RM_SAMPLING_BSDF 0u
+#define RM_SAMPLING_NEE 1u
+#define RM_SAMPLING_MIS 2u
+/* Tonemapping (bridge/Uniforms.hpp:11-15) */
+#define RM_TONEMAP_LINEAR 0u
+#define RM_TONEMAP_HABLE 1u
+#define RM_TONEMAP_ACES 2u
+
+const char* rm_last_error(void);
+const char* rm_version(void);
+
+/* ---- device ------------------------------------------------------------ */
+
+typedef struct rm_device_info {
+    int compute_units;
+    int wavefront_size;
+    uint64_t global_memory_bytes;
+    char name[64];
+    char arch[32];
+} rm_device_info;
+
+/* Replaces MTLCreateSystemDefaultDevice (Main.swift:22). */
+int rm_device_create(int ordinal, rm_device** out);
+void rm_device_destroy(rm_device* dev);
+int rm_device_get_info(rm_device* dev, rm_device_info* out);
+
+/* ---- scene -------------------------------------------------------------- */
+
+/* One entity: an instance of a shape (a vertex/face range of the scene
+ * arrays) under a transform (EntityBuilder.swift:66-77,97-141). */
+typedef struct rm_instance_desc {
+    uint32_t vertex_offset;
+    uint32_t vertex_count;
+    uint32_t face_offset;   /* faces index vertices relative to vertex_offset */
+    uint32_t face_count;
+    float transform[16];    /* column-major 4x4 (columns c0..c3), object -> world */
+    uint32_t visibility;    /* RM_RAY_* mask */
+    uint32_t emissive;      /* non-zero: register as a shape light (EntityBuilder.swift:103-114) */
+} rm_instance_desc;
+
+typedef struct rm_light_info {
+    uint32_t shader_index;  /* light-material index */
+    uint32_t casts_shadows;
+    uint32_t uses_mis;      /* the reference forces 0 (LightBuilder.swift:45) */
+} rm_light_info;
+
+typedef struct rm_area_light {
+    rm_light_info info;
+    float transform[12];    /* float3x4 columns t0,t1,t2 (each 4 floats) */
+    float color[3];
+    uint32_t is_circular;
+} rm_area_light;
+
+typedef struct rm_point_light {
+    rm_light_info info;
+    float location[3];
+    float radius;
+    float color[3];
+} rm_point_light;
+
+typedef struct rm_sun_light {
+    rm_light_info info;
+    float direction[3];
+    float cos_angle;
+    float color[3];
+} rm_sun_light;
+
+typedef struct rm_spot_light {
+    rm_light_info info;
+    float location[3];
+    float direction[3];
+    float radius;
+    float color[3];
+    float spot_size;        /* cos(size/2) */
+    float spot_blend;
+} rm_spot_light;
+
+typedef struct rm_camera_desc {
+    float transform[16];    /* column-major camera-to-world */
+    float near_clip;
+    float far_clip;
+    float focal_length;     /* f_mm / (film_width / 2) (SceneLoader.swift:135-136) */
+    float shift[2];
+} rm_camera_desc;
+
+typedef struct rm_scene_desc {
+    uint32_t num_vertices;
+    const float* vertices;      /* 3 per vertex, object space */
+    const float* normals;       /* 3 per vertex */
+    const float* texcoords;     /* 2 per vertex, may be NULL */
+    uint32_t num_faces;
+    const uint32_t* indices;    /* 3 per face, relative to the instance's vertex_offset */
+    const uint16_t* face_materials; /* surface-material index per face */
+    uint32_t num_instances;
+    const rm_instance_desc* instances;
+    uint32_t num_area_lights;
+    const rm_area_light* area_lights;
+    uint32_t num_point_lights;
+    const rm_point_light* point_lights;
+    uint32_t num_sun_lights;
+    const rm_sun_light* sun_lights;
+    uint32_t num_spot_lights;
+    const rm_spot_light* spot_lights;
+    uint32_t world_shader;      /* light-material index of the world (LightBuilder.swift:247-248) */
+    uint32_t envmap_resolution; /* 0 = 2048 (LightBuilder.swift:90) */
+    rm_camera_desc camera;
+} rm_scene_desc;
+
+/* Copies the description, uploads geometry and light tables
+ * (ShapeBuilder.build, EntityBuilder.build, LightBuilder.build minus envmap). */
+int rm_scene_create(rm_device* dev, const rm_scene_desc* desc, rm_scene** out);
+void rm_scene_destroy(rm_scene* scene);
+/* Builds the BVH8 over all instances (replaces the Metal primitive and
+ * instance acceleration structures, ShapeBuilder.swift:7-94, EntityBuilder.swift:146-165). */
+int rm_scene_build_accel(rm_scene* scene, int host_threads);
+/* Compiles generated material source (shade_surface / shade_light switch
+ * tables) together with the shading kernels via hiprtc.  `log` receives the
+ * compiler log (may be NULL). (MaterialBuilder.swift:253-325) */
+int rm_scene_compile_materials(rm_scene* scene, const char* source, size_t length, const char* options, char* log,
+                               size_t log_capacity);
+/* Builds the importance map of the world light (LightBuilder.swift:83-167). */
+int rm_scene_build_envmap(rm_scene* scene);
+int rm_scene_set_camera(rm_scene* scene, const rm_camera_desc* camera);
+
+typedef struct rm_scene_stats {
+    uint64_t triangles;
+    uint64_t bvh_nodes;
+    uint32_t bvh_depth;
+    double bvh_build_seconds;
+    double jit_seconds;
+    double envmap_seconds;
+    uint64_t device_bytes;
+} rm_scene_stats;
+int rm_scene_get_stats(rm_scene* scene, rm_scene_stats* out);
+
+/* Reads back the envmap tables (resolution^2 pdfs, sum_k 4^k mip floats). */
+int rm_scene_read_envmap(rm_scene* scene, float* pdfs, size_t pdf_count, float* mipmap, size_t mip_count);
+/* Reads back the shape-light face CDFs (distribution.m:7-39) and per-light emissive areas. */
+int rm_scene_read_light_tables(rm_scene* scene, float* light_faces, size_t face_count, float* emissive_areas,
+                               size_t light_count);
+
+/* Raw batch traversal for parity tests (raytrace.metal:11-85).  rays: n x 8
+ * floats (ox, oy, oz, tmin, dx, dy, dz, tmax).  Outputs: t (< 0 on miss),
+ * instance, primitive, barycentrics (2 per ray: weights of vertex 0 and 1). */
+int rm_trace_closest(rm_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* inst, uint32_t* prim,
+                     float* bary);
+int rm_trace_any(rm_scene* scene, const float* rays, uint32_t n, uint32_t* occluded);
+
+/* ---- renderer ----------------------------------------------------------- */
+
+typedef struct rm_uniforms {
+    uint32_t sampling_mode;   /* RM_SAMPLING_*; default MIS */
+    uint32_t accumulate;      /* default 1 */
+    float exposure;           /* default 1 */
+    uint32_t tonemapping;     /* RM_TONEMAP_*; default linear */
+    uint32_t rr_depth;        /* Russian roulette from this depth; 0 = reference behaviour */
+    uint32_t accumulate_f64;  /* also accumulate every launch into an FP64 buffer */
+    uint32_t timing;          /* record per-stage GPU times (hipEvents) */
+    uint32_t traversal_stats; /* count BVH nodes / triangles visited (slower) */
+} rm_uniforms;
+
+typedef struct rm_render_stats {
+    uint64_t rays[33];          /* closest-hit rays traced per depth (rayCounts) */
+    uint64_t shadow_rays[32];   /* shadow rays traced per depth (shadowRayCounts) */
+    uint64_t total_rays;        /* sum of both */
+    uint64_t samples;           /* paths started */
+    double ms_total;            /* wall time of the rm_render call */
+    double ms_trace_closest;    /* stage GPU times when timing != 0 */
+    double ms_shade;
+    double ms_trace_any;
+    uint32_t trace_closest_launches;
+    uint32_t trace_any_launches;
+    uint64_t nodes_closest, tris_closest, nodes_any, tris_any; /* traversal_stats */
+} rm_render_stats;
+
+/* Replaces Renderer.init + setSize: samples_per_launch paths per pixel are
+ * traced together in one wavefront. */
+int rm_renderer_create(rm_scene* scene, uint32_t width, uint32_t height, uint32_t max_depth,
+                       uint32_t samples_per_launch, rm_renderer** out);
+void rm_renderer_destroy(rm_renderer* r);
+int rm_renderer_set_uniforms(rm_renderer* r, const rm_uniforms* u);
+/* Clears the accumulation and the frame counter (Renderer.reset). */
+int rm_renderer_reset(rm_renderer* r);
+/* Renders `samples` more samples per pixel (Renderer.execute, one call per
+ * launch of samples_per_launch).  Blocks until the GPU is done. */
+int rm_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
+/* Accumulated RGBA sums (alpha counts contributions) -- width*height*4 floats. */
+int rm_read_image(rm_renderer* r, float* rgba, size_t count);
+int rm_read_image_f64(rm_renderer* r, double* rgba, size_t count);
+/* exposure, / frames, tonemap (normalizeImage.metal:39-59) -- width*height*4 floats. */
+int rm_resolve(rm_renderer* r, float* rgba, size_t count);
+uint32_t rm_renderer_frame_index(rm_renderer* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,34 @@
+/*
+ * raymond-amd host-side helpers above the C ABI (include/rm.h).
+ *
+ * The reference keeps material code generation in its Swift host
+ * (host/scene/MaterialBuilder.swift:32-904); this is the C++ restatement a
+ * host binds next to rm.h.  Input: the Blender exporter's node-graph JSON
+ * (scripts/blender_exporter/raymond_blender/materials.py:300-560) wrapped as
+ *   {"surface": [{"name": ..., "nodes": {...}}, ...],
+ *    "light":   [{"name": ..., "nodes": {...}}, ...]}
+ * where array position is the material index.  Output: source text defining
+ * rm::shade_surface and rm::shade_light, to pass to rm_scene_compile_materials.
+ */
+#ifndef RAYMOND_AMD_RM_HOST_H
+#define RAYMOND_AMD_RM_HOST_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Returns 0 and a malloc'd source string, or -1 and a malloc'd error message
+ * (cycle, multi-input link, unsupported node; MaterialBuilder.swift:45-51).
+ * noinline != 0 emits material functions as out-of-line calls. */
+int rmh_codegen(const char* materials_json, int noinline, char** out_source, char** out_error);
+/* Identifier the codegen derives from a node, socket or material name
+ * ("Base Color" -> "baseColor", MaterialBuilder.swift:19-43). */
+int rmh_identifier(const char* name, char* out, size_t capacity);
+void rmh_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+"""ctypes binding of the C ABI (include/rm.h, include/rm_host.h).
+
+Loads raymond_amd/libraymond_amd.so built in-tree.  There is no fallback:
+if the library is missing the import of this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libraymond_amd.so")
+
+RAY_CAMERA, RAY_REFLECTION, RAY_TRANSMISSION, RAY_SHADOW = 1, 2, 4, 8
+RAY_VOLUME, RAY_DIFFUSE, RAY_GLOSSY, RAY_SINGULAR, RAY_ALL = 16, 32, 64, 128, 255
+SAMPLING_BSDF, SAMPLING_NEE, SAMPLING_MIS = 0, 1, 2
+TONEMAP_LINEAR, TONEMAP_HABLE, TONEMAP_ACES = 0, 1, 2
+
+
+class RmError(RuntimeError):
+    pass
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("compute_units", C.c_int), ("wavefront_size", C.c_int), ("global_memory_bytes", C.c_uint64),
+                ("name", C.c_char * 64), ("arch", C.c_char * 32)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("vertex_offset", C.c_uint32), ("vertex_count", C.c_uint32), ("face_offset", C.c_uint32),
+                ("face_count", C.c_uint32), ("transform", C.c_float * 16), ("visibility", C.c_uint32),
+                ("emissive", C.c_uint32)]
+
+
+class LightInfo(C.Structure):
+    _fields_ = [("shader_index", C.c_uint32), ("casts_shadows", C.c_uint32), ("uses_mis", C.c_uint32)]
+
+
+class AreaLight(C.Structure):
+    _fields_ = [("info", LightInfo), ("transform", C.c_float * 12), ("color", C.c_float * 3),
+                ("is_circular", C.c_uint32)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("info", LightInfo), ("location", C.c_float * 3), ("radius", C.c_float), ("color", C.c_float * 3)]
+
+
+class SunLight(C.Structure):
+    _fields_ = [("info", LightInfo), ("direction", C.c_float * 3), ("cos_angle", C.c_float),
+                ("color", C.c_float * 3)]
+
+
+class SpotLight(C.Structure):
+    _fields_ = [("info", LightInfo), ("location", C.c_float * 3), ("direction", C.c_float * 3),
+                ("radius", C.c_float), ("color", C.c_float * 3), ("spot_size", C.c_float),
+                ("spot_blend", C.c_float)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("near_clip", C.c_float), ("far_clip", C.c_float),
+                ("focal_length", C.c_float), ("shift", C.c_float * 2)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+        ("texcoords", C.POINTER(C.c_float)),
+        ("num_faces", C.c_uint32), ("indices", C.POINTER(C.c_uint32)), ("face_materials", C.POINTER(C.c_uint16)),
+        ("num_instances", C.c_uint32), ("instances", C.POINTER(InstanceDesc)),
+        ("num_area_lights", C.c_uint32), ("area_lights", C.POINTER(AreaLight)),
+        ("num_point_lights", C.c_uint32), ("point_lights", C.POINTER(PointLight)),
+        ("num_sun_lights", C.c_uint32), ("sun_lights", C.POINTER(SunLight)),
+        ("num_spot_lights", C.c_uint32), ("spot_lights", C.POINTER(SpotLight)),
+        ("world_shader", C.c_uint32), ("envmap_resolution", C.c_uint32),
+        ("camera", CameraDesc),
+    ]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("triangles", C.c_uint64), ("bvh_nodes", C.c_uint64), ("bvh_depth", C.c_uint32),
+                ("bvh_build_seconds", C.c_double), ("jit_seconds", C.c_double), ("envmap_seconds", C.c_double),
+                ("device_bytes", C.c_uint64)]
+
+
+class Uniforms(C.Structure):
+    _fields_ = [("sampling_mode", C.c_uint32), ("accumulate", C.c_uint32), ("exposure", C.c_float),
+                ("tonemapping", C.c_uint32), ("rr_depth", C.c_uint32), ("accumulate_f64", C.c_uint32),
+                ("timing", C.c_uint32), ("traversal_stats", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64 * 33), ("shadow_rays", C.c_uint64 * 32), ("total_rays", C.c_uint64),
+                ("samples", C.c_uint64), ("ms_total", C.c_double), ("ms_trace_closest", C.c_double),
+                ("ms_shade", C.c_double), ("ms_trace_any", C.c_double),
+                ("trace_closest_launches", C.c_uint32), ("trace_any_launches", C.c_uint32),
+                ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_any", C.c_uint64),
+                ("tris_any", C.c_uint64)]
+
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "rm_last_error": (C.c_char_p, []),
+    "rm_version": (C.c_char_p, []),
+    "rm_device_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rm_device_destroy": (None, [C.c_void_p]),
+    "rm_device_get_info": (C.c_int, [C.c_void_p, C.POINTER(DeviceInfo)]),
+    "rm_scene_create": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "rm_scene_destroy": (None, [C.c_void_p]),
+    "rm_scene_build_accel": (C.c_int, [C.c_void_p, C.c_int]),
+    "rm_scene_compile_materials": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p,
+                                             C.c_size_t]),
+    "rm_scene_build_envmap": (C.c_int, [C.c_void_p]),
+    "rm_scene_set_camera": (C.c_int, [C.c_void_p, C.POINTER(CameraDesc)]),
+    "rm_scene_get_stats": (C.c_int, [C.c_void_p, C.POINTER(SceneStats)]),
+    "rm_scene_read_envmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
+    "rm_scene_read_light_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
+    "rm_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "rm_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "rm_renderer_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
+    "rm_renderer_destroy": (None, [C.c_void_p]),
+    "rm_renderer_set_uniforms": (C.c_int, [C.c_void_p, C.POINTER(Uniforms)]),
+    "rm_renderer_reset": (C.c_int, [C.c_void_p]),
+    "rm_render": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(RenderStats)]),
+    "rm_read_image": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "rm_read_image_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "rm_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "rm_renderer_frame_index": (C.c_uint32, [C.c_void_p]),
+    "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+    "rmh_free": (None, [C.c_void_p]),
+}
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError("raymond_amd native library not built: %s (run raymond_amd/build.py)" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = load_library()
+
+
+def _check(status):
+    if status != 0:
+        raise RmError("%d: %s" % (status, LIB.rm_last_error().decode(errors="replace")))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def codegen(materials_json, noinline=False):
+    """Node-graph JSON -> material source (MaterialBuilder.swift:32-904)."""
+    src = C.c_void_p()
+    err = C.c_void_p()
+    status = LIB.rmh_codegen(materials_json.encode(), int(noinline), C.byref(src), C.byref(err))
+    if status != 0:
+        msg = C.cast(err, C.c_char_p).value.decode() if err.value else "codegen failed"
+        LIB.rmh_free(err)
+        raise RmError(msg)
+    out = C.cast(src, C.c_char_p).value.decode()
+    LIB.rmh_free(src)
+    return out
+
+
+def identifier(name):
+    buf = C.create_string_buffer(256)
+    _check(LIB.rmh_identifier(name.encode(), buf, 256))
+    return buf.value.decode()
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        self.handle = C.c_void_p()
+        _check(LIB.rm_device_create(ordinal, C.byref(self.handle)))
+
+    def info(self):
+        i = DeviceInfo()
+        _check(LIB.rm_device_get_info(self.handle, C.byref(i)))
+        return i
+
+    def close(self):
+        if self.handle:
+            LIB.rm_device_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class Scene:
+    """A device-resident scene (the reference's Scene, SceneLoader.swift:6-27)."""
+
+    def __init__(self, device, desc):
+        self.device = device
+        self.desc = desc
+        self.handle = C.c_void_p()
+        sd = desc.to_c()
+        _check(LIB.rm_scene_create(device.handle, C.byref(sd), C.byref(self.handle)))
+
+    def build_accel(self, threads=0):
+        _check(LIB.rm_scene_build_accel(self.handle, threads))
+
+    def compile_materials(self, source, options=""):
+        log = C.create_string_buffer(1 << 16)
+        b = source.encode()
+        status = LIB.rm_scene_compile_materials(self.handle, b, len(b), options.encode(), log, len(log))
+        if status != 0:
+            raise RmError("material compile failed: %s\n%s" % (LIB.rm_last_error().decode(), log.value.decode()))
+        return log.value.decode()
+
+    def build_envmap(self):
+        _check(LIB.rm_scene_build_envmap(self.handle))
+
+    def stats(self):
+        s = SceneStats()
+        _check(LIB.rm_scene_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def read_envmap(self):
+        res = self.desc.envmap_resolution or 2048
+        levels = int(np.log2(res))
+        mip = np.zeros(sum(4 ** k for k in range(levels + 1)), np.float32)
+        pdfs = np.zeros(res * res, np.float32)
+        _check(LIB.rm_scene_read_envmap(self.handle, pdfs.ctypes.data, pdfs.size, mip.ctypes.data, mip.size))
+        return pdfs, mip
+
+    def read_light_tables(self, n_faces, n_lights):
+        faces = np.zeros(max(n_faces, 1), np.float32)
+        areas = np.zeros(max(n_lights, 1), np.float32)
+        _check(LIB.rm_scene_read_light_tables(self.handle, faces.ctypes.data, faces.size, areas.ctypes.data,
+                                              areas.size))
+        return faces[:n_faces], areas[:n_lights]
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        inst = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        bary = np.zeros((n, 2), np.float32)
+        _check(LIB.rm_trace_closest(self.handle, rays.ctypes.data, n, t.ctypes.data, inst.ctypes.data,
+                                    prim.ctypes.data, bary.ctypes.data))
+        return t, inst, prim, bary
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint32)
+        _check(LIB.rm_trace_any(self.handle, rays.ctypes.data, n, occ.ctypes.data))
+        return occ
+
+    def close(self):
+        if self.handle:
+            LIB.rm_scene_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class Renderer:
+    """Wavefront renderer (host/Renderer.swift): execute/setSize/reset/saveFrame."""
+
+    def __init__(self, scene, width, height, max_depth=8, samples_per_launch=1):
+        self.scene = scene
+        self.width, self.height = width, height
+        self.handle = C.c_void_p()
+        _check(LIB.rm_renderer_create(scene.handle, width, height, max_depth, samples_per_launch,
+                                      C.byref(self.handle)))
+        self.uniforms = Uniforms(sampling_mode=SAMPLING_MIS, accumulate=1, exposure=1.0,
+                                 tonemapping=TONEMAP_LINEAR, rr_depth=0)
+        self.set_uniforms()
+
+    def set_uniforms(self, **kw):
+        for k, v in kw.items():
+            setattr(self.uniforms, k, v)
+        _check(LIB.rm_renderer_set_uniforms(self.handle, C.byref(self.uniforms)))
+
+    def reset(self):
+        _check(LIB.rm_renderer_reset(self.handle))
+
+    def execute(self, samples=1):
+        st = RenderStats()
+        _check(LIB.rm_render(self.handle, samples, C.byref(st)))
+        return st
+
+    def image(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(LIB.rm_read_image(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def image_f64(self):
+        out = np.zeros((self.height, self.width, 4), np.float64)
+        _check(LIB.rm_read_image_f64(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def resolve(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(LIB.rm_resolve(self.handle, out.ctypes.data, out.size))
+        return out
+
+    @property
+    def frame_index(self):
+        return LIB.rm_renderer_frame_index(self.handle)
+
+    def close(self):
+        if self.handle:
+            LIB.rm_renderer_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()

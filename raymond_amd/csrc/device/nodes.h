@@ -1,0 +1,506 @@
+// Shader-node library for generated material code (device/nodes/nodes.hpp).
+// Node fields double as inputs and outputs: the generated code assigns the
+// inputs, calls compute(), and downstream nodes read output fields
+// (device/nodes/nodes.hpp, e.g. Math :1150-1157; host/scene/MaterialBuilder.swift:124-128).
+// Field names follow the reference's camelCase socket identifiers.
+#pragma once
+#include "common.h"
+#include "elem.h"
+#include "noise.h"
+#include "shading.h"
+
+namespace rm {
+
+RM_HD float luminance(vec3 c) { return (0.2126f * c.x + 0.7152f * c.y) + 0.0722f * c.z; }
+
+// (device/utils/color.hpp:3-73)
+RM_HD vec3 rgb2hsv(vec3 rgb) {
+    const float cmax = max_(rgb.x, max_(rgb.y, rgb.z));
+    const float cmin = min_(rgb.x, min_(rgb.y, rgb.z));
+    const float cdelta = cmax - cmin;
+    float h = 0.0f, s = 0.0f;
+    const float v = cmax;
+    if (cmax != 0.0f) s = cdelta / cmax;
+    if (s != 0.0f) {
+        const vec3 c = (cmax - rgb) / cdelta;
+        if (rgb.x == cmax) h = (0.0f + c.z) - c.y;
+        else if (rgb.y == cmax) h = (2.0f + c.x) - c.z;
+        else h = (4.0f + c.y) - c.x;
+        h /= 6.0f;
+        if (h < 0.0f) h += 1.0f;
+    }
+    return vec3(h, s, v);
+}
+RM_HD vec3 hsv2rgb(vec3 hsv) {
+    float h = hsv.x;
+    const float s = hsv.y, v = hsv.z;
+    if (s == 0.0f) return vec3(v, v, v);
+    if (h == 1.0f) h = 0.0f;
+    h *= 6.0f;
+    const float i = floor_(h);
+    const float f = h - i;
+    const float p = v * (1.0f - s);
+    const float q = v * (1.0f - (s * f));
+    const float t = v * (1.0f - (s * (1.0f - f)));
+    switch ((int)i) {
+    case 0: return vec3(v, t, p);
+    case 1: return vec3(q, v, p);
+    case 2: return vec3(p, v, t);
+    case 3: return vec3(p, q, v);
+    case 4: return vec3(t, p, v);
+    default: return vec3(v, p, q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Socket type coercions (device/nodes/nodes.hpp:2007-2031)
+RM_HD vec3 VECTOR(float v) { return vec3(v); }
+RM_HD vec3 VECTOR(vec2 v) { return vec3(v, 0.0f); }
+RM_HD vec3 VECTOR(vec3 v) { return v; }
+RM_HD vec3 VECTOR(vec4 v) { return v.xyz(); }
+RM_HD vec3 VECTOR(const UberShader&) { return vec3(0.0f); }
+RM_HD vec4 RGBA(float v) { return vec4(v, v, v, 1.0f); }
+RM_HD vec4 RGBA(vec2 v) { return vec4(v.x, v.y, 0.0f, 1.0f); }
+RM_HD vec4 RGBA(vec3 v) { return vec4(v, 1.0f); }
+RM_HD vec4 RGBA(vec4 v) { return v; }
+RM_HD vec4 RGBA(const UberShader& v) { return vec4(v.emission, 1.0f); }
+RM_HD float VALUE(float v) { return v; }
+RM_HD float VALUE(vec3 v) { return ((v.x + v.y) + v.z) / 3.0f; }
+RM_HD float VALUE(vec4 v) { return (v.w * ((v.x + v.y) + v.z)) / 3.0f; }
+RM_HD UberShader SHADER(const UberShader& v) { return v; }
+RM_HD UberShader SHADER(vec3 v) { UberShader m; m.emission = v; return m; }
+RM_HD UberShader SHADER(float v) { return SHADER(vec3(v)); }
+RM_HD UberShader SHADER(vec2 v) { return SHADER(vec3(v, 0.0f)); }
+RM_HD UberShader SHADER(vec4 v) { return SHADER(v.xyz()); }
+
+// ---------------------------------------------------------------------------
+// Inputs / geometry
+
+struct Value {
+    float value = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+struct RGB {
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+struct TextureCoordinate {
+    vec3 generated, uv, object, normal, reflection;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        uv = s.uv;
+        generated = s.generated;
+        object = s.object;
+        normal = s.normal;
+        reflection = s.normal;
+    }
+};
+
+struct NewGeometry {
+    vec3 normal, trueNormal, tangent, position, parametric, incoming, randomPerIsland;
+    bool backfacing = false;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        normal = s.normal;
+        trueNormal = s.true_normal;
+        tangent = s.tu;
+        position = s.position;
+        parametric = s.uv;
+        incoming = s.wo;
+        randomPerIsland = vec3(0.0f);
+        backfacing = dot(s.wo, s.normal) < 0.0f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Textures
+
+struct kTexNoise { enum Dimension { DIMENSION_1D, DIMENSION_2D, DIMENSION_3D, DIMENSION_4D }; };
+
+template <int Dimension>
+struct TexNoise {
+    float detail = 0.0f, distortion = 0.0f, roughness = 0.0f, scale = 0.0f, w = 0.0f;
+    vec3 vector;
+    float fac = 0.0f;
+    vec4 color;
+
+    RM_HD static float off1(float seed) { return 100.0f + hash_float_to_float(seed) * 100.0f; }
+    RM_HD static vec2 off2(float seed) {
+        return vec2(100.0f + hash_float2_to_float(vec2(seed, 0.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 1.0f)) * 100.0f);
+    }
+    RM_HD static vec3 off3(float seed) {
+        return vec3(100.0f + hash_float2_to_float(vec2(seed, 0.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 1.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 2.0f)) * 100.0f);
+    }
+
+    RM_HD void tex1(float p) {
+        if (distortion != 0.0f) p += snoise1(p + off1(0.0f)) * distortion;
+        color = vec4(fractal_noise(p, detail, roughness), fractal_noise(p + off1(1.0f), detail, roughness),
+                     fractal_noise(p + off1(2.0f), detail, roughness), 1.0f);
+        fac = color.x;
+    }
+    RM_HD void tex2(vec2 p) {
+        if (distortion != 0.0f) p = p + distortion * vec2(snoise2(p + off2(0.0f)), snoise2(p + off2(1.0f)));
+        color = vec4(fractal_noise(p, detail, roughness), fractal_noise(p + off2(2.0f), detail, roughness),
+                     fractal_noise(p + off2(3.0f), detail, roughness), 1.0f);
+        fac = color.x;
+    }
+    RM_HD void tex3(vec3 p) {
+        if (distortion != 0.0f)
+            p = p + distortion * vec3(snoise3(p + off3(0.0f)), snoise3(p + off3(1.0f)), snoise3(p + off3(2.0f)));
+        color = vec4(fractal_noise(p, detail, roughness), fractal_noise(p + off3(3.0f), detail, roughness),
+                     fractal_noise(p + off3(4.0f), detail, roughness), 1.0f);
+        fac = color.x;
+    }
+
+    // (device/nodes/nodes.hpp:881-1011); the 4-D variant falls back to 3-D
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 p = vector * scale;
+        const float ws = w * scale;
+        switch (Dimension) {
+        case kTexNoise::DIMENSION_1D: tex1(ws); break;
+        case kTexNoise::DIMENSION_2D: tex2(p.xy()); break;
+        default: tex3(p); break;
+        }
+    }
+};
+
+// Stub, as in the reference: colour 1, distance 0 (device/nodes/nodes.hpp:590-606)
+struct TexVoronoi {
+    float exponent = 0.0f, randomness = 0.0f, scale = 0.0f, smoothness = 0.0f;
+    vec3 vector;
+    float w = 0.0f;
+    vec4 color;
+    float distance = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        color = vec4(1.0f);
+        distance = 0.0f;
+    }
+};
+
+// Analytic sky: a zenith-horizon gradient plus an exponential sun lobe.
+// Not a reference node (the reference's Nishita sky is a precomputed
+// texture); `vector` defaults to the direction into the sky.
+struct TexSkyAnalytic {
+    vec3 vector;
+    vec3 sunDirection = vec3(0.0f, 0.0f, 1.0f);
+    float sunStrength = 0.0f;
+    float sunSharpness = 1.0f;
+    vec4 zenithColor, horizonColor, groundColor;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 d = normalize(vector);
+        vec3 c;
+        if (d.z >= 0.0f) {
+            const float t = sqrt_(d.z);
+            c = (1.0f - t) * horizonColor.xyz() + t * zenithColor.xyz();
+        } else {
+            const float t = min_(1.0f, -4.0f * d.z);
+            c = (1.0f - t) * horizonColor.xyz() + t * groundColor.xyz();
+        }
+        const float cs = dot(d, normalize(sunDirection));
+        const float sun = sunStrength * exp_(sunSharpness * (cs - 1.0f));
+        color = vec4(c + vec3(sun), 1.0f);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Converters
+
+template <int NumElements>
+struct ColorRamp {
+    struct Element {
+        float position;
+        vec4 color;
+    };
+    float fac = 0.0f;
+    vec4 color;
+    Element elements[NumElements];
+
+    // linear interpolation between stops (device/nodes/nodes.hpp:1013-1046)
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        if (fac < elements[0].position) { color = elements[0].color; return; }
+        if (fac > elements[NumElements - 1].position) { color = elements[NumElements - 1].color; return; }
+        for (int i = 1; i < NumElements; ++i) {
+            if (elements[i].position >= fac) {
+                const Element a = elements[i - 1], b = elements[i];
+                const float v = (fac - a.position) / (b.position - a.position);
+                color = (1.0f - v) * a.color + v * b.color;
+                return;
+            }
+        }
+    }
+};
+
+struct kMath {
+    enum Operation {
+        OPERATION_ADD, OPERATION_SUBTRACT, OPERATION_MULTIPLY, OPERATION_DIVIDE, OPERATION_MULTIPLY_ADD,
+        OPERATION_POWER, OPERATION_MINIMUM, OPERATION_MAXIMUM, OPERATION_TANGENT, OPERATION_LESS_THAN,
+        OPERATION_GREATER_THAN, OPERATION_MODULO
+    };
+};
+
+RM_HD float fmod_(float a, float b) {
+#if defined(RM_GPU)
+    return __builtin_fmodf(a, b);
+#else
+    return fmodf(a, b);
+#endif
+}
+
+// (device/nodes/nodes.hpp:1145-1193)
+template <int Operation, bool Clamp>
+struct Math {
+    float value = 0.0f, value_001 = 0.0f, value_002 = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        switch (Operation) {
+        case kMath::OPERATION_ADD: value = value + value_001; break;
+        case kMath::OPERATION_SUBTRACT: value = value - value_001; break;
+        case kMath::OPERATION_MULTIPLY: value = value * value_001; break;
+        case kMath::OPERATION_DIVIDE: value = safe_divide(value, value_001, 0.0f); break;
+        case kMath::OPERATION_MULTIPLY_ADD: value = value * value_001 + value_002; break;
+        case kMath::OPERATION_POWER: value = pow_(value, value_001); break;
+        case kMath::OPERATION_MINIMUM: value = value < value_001 ? value : value_001; break;
+        case kMath::OPERATION_MAXIMUM: value = value > value_001 ? value : value_001; break;
+        case kMath::OPERATION_TANGENT: value = tan_(value); break;
+        case kMath::OPERATION_LESS_THAN: value = value < value_001 ? 1.0f : 0.0f; break;
+        case kMath::OPERATION_GREATER_THAN: value = value > value_001 ? 1.0f : 0.0f; break;
+        case kMath::OPERATION_MODULO: value = fmod_(value, value_001); break;
+        }
+        if (Clamp) value = saturate(value);
+    }
+};
+
+struct kColorMix {
+    enum BlendType {
+        BLEND_TYPE_MIX, BLEND_TYPE_ADD, BLEND_TYPE_MULTIPLY, BLEND_TYPE_SCREEN, BLEND_TYPE_OVERLAY,
+        BLEND_TYPE_SUB, BLEND_TYPE_COLOR, BLEND_TYPE_LIGHTEN, BLEND_TYPE_DARKEN, BLEND_TYPE_VALUE
+    };
+};
+
+RM_HD vec4 vmin4(vec4 a, vec4 b) { return vec4(min_(a.x, b.x), min_(a.y, b.y), min_(a.z, b.z), min_(a.w, b.w)); }
+RM_HD vec4 vmax4(vec4 a, vec4 b) { return vec4(max_(a.x, b.x), max_(a.y, b.y), max_(a.z, b.z), max_(a.w, b.w)); }
+
+// (device/nodes/nodes.hpp:1319-1385); SCREEN uses color1 twice as in the reference
+template <int BlendType, bool Clamp>
+struct ColorMix {
+    vec4 color1, color2;
+    float fac = 0.0f;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        switch (BlendType) {
+        case kColorMix::BLEND_TYPE_MIX: color = lerp(color1, color2, fac); break;
+        case kColorMix::BLEND_TYPE_ADD: color = color1 + fac * color2; break;
+        case kColorMix::BLEND_TYPE_SUB: color = color1 - fac * color2; break;
+        case kColorMix::BLEND_TYPE_MULTIPLY: color = color1 * lerp(vec4(1.0f), color2, fac); break;
+        case kColorMix::BLEND_TYPE_SCREEN: color = 1.0f - (1.0f - fac * color1) * (1.0f - color1); break;
+        case kColorMix::BLEND_TYPE_OVERLAY: {
+            color = color1;
+            float* c = &color.x;
+            const float* c2 = &color2.x;
+            for (int dim = 0; dim < 3; dim++) {
+                if (c[dim] < 0.5f) c[dim] *= (1.0f - fac) + (2.0f * fac) * c2[dim];
+                else c[dim] = 1.0f - ((1.0f - fac) + (2.0f * fac) * (1.0f - c2[dim])) * (1.0f - c[dim]);
+            }
+            break;
+        }
+        case kColorMix::BLEND_TYPE_COLOR: {
+            color = color1;
+            const vec3 hsv2 = rgb2hsv(color2.xyz());
+            if (hsv2.y == 0.0f) {
+                color = color1;
+            } else {
+                vec3 hsv = rgb2hsv(color1.xyz());
+                hsv.x = hsv2.x;
+                hsv.y = hsv2.y;
+                color = lerp(color1, vec4(hsv2rgb(hsv), color2.w), fac);
+            }
+            break;
+        }
+        case kColorMix::BLEND_TYPE_LIGHTEN: color = lerp(color1, vmax4(color1, color2), fac); break;
+        case kColorMix::BLEND_TYPE_DARKEN: color = lerp(color1, vmin4(color1, color2), fac); break;
+        case kColorMix::BLEND_TYPE_VALUE: {
+            vec3 hsv = rgb2hsv(color1.xyz());
+            hsv.z = rgb2hsv(color2.xyz()).z;
+            color = lerp(color1, vec4(hsv2rgb(hsv), color2.w), fac);
+            break;
+        }
+        }
+        if (Clamp) color = saturate(color);
+    }
+};
+
+// No-op, as in the reference: `normal` passes its input through (device/nodes/nodes.hpp:294-302)
+struct Bump {
+    float distance = 0.0f, height = 0.0f;
+    vec3 normal;
+    float strength = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+// ---------------------------------------------------------------------------
+// Shaders
+
+struct kBsdfPrincipled {
+    enum Distribution { DISTRIBUTION_GGX };
+    enum SubsurfaceMethod { SUBSURFACE_METHOD_BURLEY, SUBSURFACE_METHOD_RANDOM_WALK, SUBSURFACE_METHOD_RANDOM_WALK_FIXED_RADIUS };
+};
+
+// Disney parameters to lobe weights (device/nodes/nodes.hpp:1581-1687)
+template <int Distribution, int SubsurfaceMethod>
+struct BsdfPrincipled {
+    float weight = 0.0f;
+    vec4 emission;
+    float sheenTint = 0.0f, emissionStrength = 0.0f, transmission = 0.0f;
+    vec3 clearcoatNormal;
+    float alpha = 0.0f, specularTint = 0.0f;
+    vec3 tangent;
+    float roughness = 0.0f, subsurfaceIor = 0.0f, anisotropic = 0.0f, sheen = 0.0f;
+    vec3 subsurfaceRadius, normal;
+    float subsurfaceAnisotropy = 0.0f;
+    vec4 baseColor;
+    float transmissionRoughness = 0.0f, metallic = 0.0f, specular = 0.0f, clearcoatRoughness = 0.0f;
+    vec4 subsurfaceColor;
+    float subsurface = 0.0f, ior = 0.0f, anisotropicRotation = 0.0f, clearcoat = 0.0f;
+    UberShader bsdf;
+
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        clearcoatRoughness = max_(clearcoatRoughness, 1e-4f);
+        roughness = max_(roughness, 1e-4f);
+        const float diffuse_w = (1.0f - saturate(transmission)) * (1.0f - saturate(metallic));
+        const float transmission_w = saturate(transmission) * (1.0f - saturate(metallic));
+        const float specular_w = 1.0f - transmission_w;
+        const vec3 base = baseColor.xyz();
+        const float lum = luminance(base);
+        const vec3 tint = lum > 0.0f ? base * (1.0f / lum) : vec3(1.0f);
+        const vec3 sheen_color = lerp(vec3(1.0f), tint, sheenTint);
+        const vec3 specular_color = lerp(vec3(1.0f), tint, specularTint);
+        const vec3 cspec0 = lerp((specular * 0.08f) * specular_color, base, metallic);
+        const float aspect = sqrt_(1.0f - 0.9f * anisotropic);
+        const float r2 = square(roughness);
+
+        bsdf.diffuse.diffuse_weight = diffuse_w * base;
+        bsdf.diffuse.sheen_weight = (diffuse_w * sheen) * sheen_color;
+        bsdf.diffuse.roughness = roughness;
+        bsdf.diffuse.translucent = false;
+
+        bsdf.specular.alpha_x = r2 / aspect;
+        bsdf.specular.alpha_y = r2 * aspect;
+        bsdf.specular.cspec0 = cspec0;
+        bsdf.specular.ior = (2.0f / (1.0f - sqrt_(0.08f * specular))) - 1.0f;
+        bsdf.specular.weight = specular_w;
+
+        bsdf.transmission.reflection_alpha = r2;
+        bsdf.transmission.transmission_alpha = square(1.0f - (1.0f - roughness) * (1.0f - transmissionRoughness));
+        bsdf.transmission.base_color = base;
+        bsdf.transmission.cspec0 = lerp(vec3(1.0f), base, specularTint);
+        bsdf.transmission.ior = ior;
+        bsdf.transmission.weight = transmission_w;
+        bsdf.transmission.only_refract = false;
+
+        bsdf.clearcoat.alpha = square(clearcoatRoughness);
+        bsdf.clearcoat.weight = clearcoat;
+
+        bsdf.lobe_prob[0] = diffuse_w;
+        bsdf.lobe_prob[1] = specular_w;
+        bsdf.lobe_prob[2] = transmission_w;
+        bsdf.lobe_prob[3] = clearcoat * 0.25f;
+        float sum = 0.0f;
+        for (int i = 0; i < 4; i++) sum += bsdf.lobe_prob[i];
+        for (int i = 0; i < 4; i++) bsdf.lobe_prob[i] /= sum;
+
+        bsdf.alpha = alpha;
+        bsdf.normal = normal;
+        bsdf.emission = (alpha * emission.xyz()) * emissionStrength;
+    }
+};
+
+// Burley diffuse (device/nodes/nodes.hpp:1809-1827)
+struct BsdfDiffuse {
+    vec4 color;
+    vec3 normal;
+    float roughness = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        bsdf.diffuse.diffuse_weight = color.xyz();
+        bsdf.diffuse.sheen_weight = vec3(0.0f);
+        bsdf.diffuse.roughness = roughness;
+        bsdf.diffuse.translucent = false;
+        bsdf.normal = normal;
+        bsdf.lobe_prob[0] = 1.0f;
+    }
+};
+
+// (device/nodes/nodes.hpp:1396-1407)
+struct Emission {
+    vec4 color;
+    float strength = 0.0f, weight = 0.0f;
+    UberShader emission;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        emission.lobe_prob[0] = 1.0f;
+        emission.emission = color.xyz() * strength;
+    }
+};
+
+// (device/nodes/nodes.hpp:1409-1419)
+struct Background {
+    vec4 color;
+    float strength = 0.0f, weight = 0.0f;
+    UberShader background;
+    RM_HD void compute(const SceneView&, ShadingContext&) { background.emission = color.xyz() * strength; }
+};
+
+// Stochastic shader mix on shading.rnd.x (device/nodes/nodes.hpp:1916-1931)
+struct MixShader {
+    float fac = 0.0f;
+    UberShader shader, shader_001;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        const vec3 em = fac * shader_001.emission + (1.0f - fac) * shader.emission;
+        if (s.rnd.x < fac) {
+            s.rnd.x = s.rnd.x / fac;
+            shader = shader_001;
+        } else {
+            s.rnd.x = (s.rnd.x - fac) / (1.0f - fac);
+        }
+        shader.emission = em;
+    }
+};
+
+// (device/nodes/nodes.hpp:1899-1913); the weight doubling is the reference's
+struct AddShader {
+    UberShader shader, shader_001;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        const vec3 em = shader_001.emission + shader.emission;
+        if (s.rnd.x < 0.5f) {
+            s.rnd.x = s.rnd.x / 0.5f;
+            shader = shader_001;
+        } else {
+            s.rnd.x = 2.0f * (s.rnd.x - 0.5f);
+        }
+        shader.emission = em;
+        shader.weight *= 2.0f;
+    }
+};
+
+// Outputs (device/nodes/nodes.hpp:1976-2005)
+struct OutputMaterial {
+    vec3 displacement;
+    float thickness = 0.0f;
+    UberShader surface, volume;
+    RM_HD void compute(const SceneView&, ShadingContext& s) { s.material = surface; }
+};
+struct OutputWorld {
+    float thickness = 0.0f;
+    UberShader surface;
+    RM_HD void compute(const SceneView&, ShadingContext& s) { s.material = surface; }
+};
+struct OutputLight {
+    UberShader surface;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        s.material.alpha = 0.0f;
+        s.material.emission = surface.emission;
+    }
+};
+
+}  // namespace rm

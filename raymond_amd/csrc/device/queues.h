@@ -1,0 +1,71 @@
+// Wavefront path state in HBM, structure-of-arrays.
+//
+// The reference keeps AoS Ray (80 B), ShadowRay (64 B) and Intersection
+// (24 B) records (bridge/Ray.hpp:17-46) in ping-pong buffers of 2 x W x H
+// (host/Renderer.swift:766-775).  Here every field is its own array so a
+// wave's 64 lanes read and write 256 contiguous bytes per field, and only
+// the fields a kernel needs are touched.
+#pragma once
+#include "common.h"
+
+namespace rm {
+
+constexpr int kMaxDepth = 32;
+
+// path meta word: RNG draw index (16) | ray flags (8) | depth (8)
+RM_HD uint32_t pack_meta(uint32_t prng_index, uint32_t flags, uint32_t depth) {
+    return (prng_index & 0xffffu) | ((flags & 0xffu) << 16) | ((depth & 0xffu) << 24);
+}
+RM_HD uint32_t meta_index(uint32_t m) { return m & 0xffffu; }
+RM_HD uint32_t meta_flags(uint32_t m) { return (m >> 16) & 0xffu; }
+RM_HD uint32_t meta_depth(uint32_t m) { return m >> 24; }
+
+struct PathQueue {
+    float *ox, *oy, *oz;
+    float *dx, *dy, *dz;
+    float *wx, *wy, *wz;
+    float* pdf;       // pdf of the BSDF sample that produced the ray (inf for camera rays)
+    uint32_t* seed;   // RNG key
+    uint32_t* meta;   // pack_meta
+    uint32_t* pixel;  // accumulation-buffer pixel index
+};
+
+struct HitQueue {
+    float* t;         // < 0 on miss
+    uint32_t* prim;
+    uint32_t* inst;
+    float* b0;
+    float* b1;
+};
+
+struct ShadowQueue {
+    float *ox, *oy, *oz;
+    float *dx, *dy, *dz;
+    float* tmax;
+    float *wx, *wy, *wz;
+    uint32_t* pixel;
+};
+
+// Per-launch device counters (zeroed by a kernel at launch start).
+struct Counters {
+    uint32_t rays[kMaxDepth + 1];      // paths entering each depth
+    uint32_t shadows[kMaxDepth];       // shadow rays emitted at each depth
+    uint32_t fetch_rays[kMaxDepth + 1];
+    uint32_t fetch_shadows[kMaxDepth];
+    uint32_t shade_done[kMaxDepth];
+    // traversal statistics (only written by the statistics variants)
+    unsigned long long nodes_closest, tris_closest, nodes_any, tris_any;
+};
+
+// Render-time settings (bridge/Uniforms.hpp:29-47 plus this build's knobs).
+struct FrameUniforms {
+    uint32_t width, height;
+    uint32_t samples;          // samples traced together in this launch
+    uint32_t random_seed;      // seed of sample 0 (frame index + 1)
+    uint32_t sampling_mode;    // 0 bsdf, 1 nee, 2 mis
+    uint32_t rr_depth;         // roulette starts at this depth (0 = reference behaviour)
+    uint32_t max_depth;
+    uint32_t image_offset;     // first pixel of this job in the accumulation buffer
+};
+
+}  // namespace rm

@@ -1,0 +1,265 @@
+// Kernels compiled at run time together with a scene's generated material
+// code (hiprtc).  Included after the material functions, so shade_surface /
+// shade_light resolve to the scene's switch tables.
+//
+//  rm_shade_kernel         handleIntersections (device/kernels/shading/intersection.metal:22-211)
+//  rm_envmap_build_kernel  buildEnvironmentMap (device/kernels/envmap/build.metal:4-51)
+#pragma once
+#include "common.h"
+#include "rng.h"
+#include "warp.h"
+#include "bsdf.h"
+#include "scene.h"
+#include "shading.h"
+#include "lights.h"
+#include "queues.h"
+#include "shade_kernels_abi.h"
+
+namespace rm {
+
+// power heuristic (device/kernels/shading/intersection.metal:14-20)
+__device__ inline float mis_weight(float pdf, float other) {
+    pdf = pdf * pdf;
+    if (isinf_(pdf)) return 1.0f;
+    other = other * other;
+    return pdf / (pdf + other);
+}
+
+// Cycles' shading-normal fix (device/utils/math.hpp:73-119)
+__device__ inline vec3 ensure_valid_reflection(vec3 Ng, vec3 I, vec3 N) {
+    const vec3 R = (2.0f * dot(N, I)) * N - I;
+    const float threshold = min_(0.9f * dot(Ng, I), 0.01f);
+    if (dot(Ng, R) >= threshold) return N;
+    const float NdotNg = dot(N, Ng);
+    const vec3 X = normalize(N - NdotNg * Ng);
+    const float Ix = dot(I, X), Iz = dot(I, Ng);
+    const float Ix2 = sqr(Ix), Iz2 = sqr(Iz);
+    const float a = Ix2 + Iz2;
+    const float b = safe_sqrtf(Ix2 * (a - sqr(threshold)));
+    const float c = Iz * threshold + a;
+    const float fac = 0.5f / a;
+    const float N1_z2 = fac * (b + c), N2_z2 = fac * (-b + c);
+    bool valid1 = (N1_z2 > 1e-5f) && (N1_z2 <= (1.0f + 1e-5f));
+    bool valid2 = (N2_z2 > 1e-5f) && (N2_z2 <= (1.0f + 1e-5f));
+    vec2 N_new;
+    if (valid1 && valid2) {
+        const vec2 N1(safe_sqrtf(1.0f - N1_z2), safe_sqrtf(N1_z2));
+        const vec2 N2(safe_sqrtf(1.0f - N2_z2), safe_sqrtf(N2_z2));
+        const float R1 = (2.0f * (N1.x * Ix + N1.y * Iz)) * N1.y - Iz;
+        const float R2 = (2.0f * (N2.x * Ix + N2.y * Iz)) * N2.y - Iz;
+        valid1 = (R1 >= 1e-5f);
+        valid2 = (R2 >= 1e-5f);
+        if (valid1 && valid2) N_new = (R1 < R2) ? N1 : N2;
+        else N_new = (R1 > R2) ? N1 : N2;
+    } else if (valid1 || valid2) {
+        const float Nz2 = valid1 ? N1_z2 : N2_z2;
+        N_new = vec2(safe_sqrtf(1.0f - Nz2), safe_sqrtf(Nz2));
+    } else {
+        return Ng;
+    }
+    return N_new.x * X + N_new.y * Ng;
+}
+
+__device__ inline void image_add(float* image, uint32_t pixel, vec3 c) {
+    float* p = image + 4u * (size_t)pixel;
+    atomicAdd(p + 0, c.x);
+    atomicAdd(p + 1, c.y);
+    atomicAdd(p + 2, c.z);
+    atomicAdd(p + 3, 1.0f);
+}
+
+// Wave-level compaction: one atomic per wave, slots in lane order.  Must be
+// called with every lane of the wave active.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    const uint32_t lane = __lane_id();
+    const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (mask != 0ull) {
+        const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
+        if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader);
+    }
+    return base + before;
+}
+
+// One path vertex (device/kernels/shading/intersection.metal:43-208).
+// Image contributions are added here; the shadow and continuation rays are
+// returned for compacted append by the caller.
+struct ShadeOut {
+    bool shadow = false;
+    vec3 s_origin, s_dir, s_weight;
+    float s_tmax = 0.0f;
+    bool next = false;
+    vec3 n_origin, n_dir, n_weight;
+    float n_pdf = 0.0f;
+    uint32_t n_meta = 0;
+    uint32_t seed = 0;
+    uint32_t pixel = 0;
+};
+
+__device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
+    const SceneView& ctx = a.ctx;
+    const uint32_t mode = a.u.sampling_mode;
+    const vec3 dir(a.cur.dx[i], a.cur.dy[i], a.cur.dz[i]);
+    const vec3 ray_weight(a.cur.wx[i], a.cur.wy[i], a.cur.wz[i]);
+    const float ray_pdf = a.cur.pdf[i];
+    const uint32_t meta = a.cur.meta[i];
+    const uint32_t flags = meta_flags(meta);
+    const uint32_t depth = meta_depth(meta);
+    const uint32_t pixel = a.cur.pixel[i];
+    Prng prng = Prng::from_state(a.cur.seed[i], (uint16_t)meta_index(meta));
+    out.pixel = pixel;
+    const bool collect_emission = isinf_(ray_pdf) || mode != 1u;
+    const float hit_t = a.hits.t[i];
+
+    ShadingContext shading;
+    shading.ray_flags = flags;
+    shading.rnd = prng.sample3d();
+    shading.wo = -dir;
+
+    if (hit_t <= 0.0f) {
+        if (collect_emission) {
+            const float mis = (mode == 0u || isinf_(ray_pdf)) ? 1.0f : mis_weight(ray_pdf, lights_envmap_pdf(ctx, dir));
+            evaluate_environment(ctx, shading);
+            image_add(a.image, pixel, (mis * ray_weight) * shading.material.emission);
+        }
+        return;
+    }
+
+    const uint32_t inst_idx = a.hits.inst[i];
+    const PerInstance& inst = ctx.instances[inst_idx];
+    uint32_t shader_index;
+    shading.build(ctx, inst, a.hits.prim[i], vec2(a.hits.b0[i], a.hits.b1[i]), hit_t, shader_index);
+    if (inst.visibility & flags) shade_surface(shader_index, ctx, shading);
+    else shading.material.alpha = 0.0f;
+
+    if (collect_emission && mean(shading.material.emission) != 0.0f) {
+        const float mis = (mode == 0u || isinf_(ray_pdf)) ? 1.0f
+                                                           : mis_weight(ray_pdf, lights_shape_pdf(ctx, inst, shading));
+        image_add(a.image, pixel, (mis * ray_weight) * shading.material.emission);
+    }
+
+    vec3 sh_normal;
+    if (all_zero(shading.material.normal)) {
+        sh_normal = shading.true_normal;
+    } else {
+        vec3 geo = shading.true_normal;
+        if (dot(geo, shading.wo) < 0.0f) geo = geo * -1.0f;
+        sh_normal = ensure_valid_reflection(geo, shading.wo, shading.material.normal);
+    }
+
+    if (mode != 0u) {
+        const LightSample nee = lights_sample(ctx, shading, prng);
+        float bsdf_pdf;
+        const vec3 bsdf = shading.material.evaluate(shading.wo, nee.direction, sh_normal, shading.true_normal, bsdf_pdf);
+        const vec3 contribution = (nee.weight * bsdf) * ray_weight;
+        if (nee.casts_shadows) {
+            const float mis = (mode == 1u || !nee.can_be_hit) ? 1.0f : mis_weight(nee.pdf, bsdf_pdf);
+            const vec3 nee_weight = mis * contribution;
+            if (all_finite(nee_weight) && any_nonzero(nee_weight)) {
+                out.shadow = true;
+                out.s_origin = shading.position;
+                out.s_dir = nee.direction;
+                out.s_tmax = nee.distance - kEps;
+                out.s_weight = nee_weight;
+            }
+        } else {
+            // the reference applies the path weight twice here (intersection.metal:155,176)
+            image_add(a.image, pixel, ray_weight * contribution);
+        }
+    }
+
+    const BsdfSample s = shading.material.sample(shading.rnd, -dir, sh_normal, shading.true_normal, flags);
+    const vec3 w = ray_weight * s.weight;
+    const float mean_w = mean(w);
+    if (!isfinite_(mean_w)) return;
+    float survival;
+    bool survive;
+    if (depth < a.u.rr_depth) {
+        survival = 1.0f;
+        survive = mean_w > 0.0f;
+    } else {
+        survival = min_(mean_w, 1.0f);
+        survive = prng.sample() < survival;
+    }
+    if (survive) {
+        out.next = true;
+        out.n_origin = shading.position;
+        out.n_dir = s.wi;
+        out.n_weight = w * (1.0f / survival);
+        out.n_pdf = s.pdf;
+        out.n_meta = pack_meta(prng.index, s.flags, depth + 1);
+        out.seed = prng.seed;
+    }
+}
+
+extern "C" __global__ __launch_bounds__(256) void rm_shade_kernel(ShadeArgs a) {
+    const uint32_t count = a.counters->rays[a.depth];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        ShadeOut out;
+        if (i < count) shade_path(a, i, out);
+        const uint32_t sslot = wave_append(&a.counters->shadows[a.depth], out.shadow);
+        if (out.shadow) {
+            a.shadows.ox[sslot] = out.s_origin.x;
+            a.shadows.oy[sslot] = out.s_origin.y;
+            a.shadows.oz[sslot] = out.s_origin.z;
+            a.shadows.dx[sslot] = out.s_dir.x;
+            a.shadows.dy[sslot] = out.s_dir.y;
+            a.shadows.dz[sslot] = out.s_dir.z;
+            a.shadows.tmax[sslot] = out.s_tmax;
+            a.shadows.wx[sslot] = out.s_weight.x;
+            a.shadows.wy[sslot] = out.s_weight.y;
+            a.shadows.wz[sslot] = out.s_weight.z;
+            a.shadows.pixel[sslot] = out.pixel;
+        }
+        const uint32_t nslot = wave_append(&a.counters->rays[a.depth + 1], out.next);
+        if (out.next) {
+            a.next.ox[nslot] = out.n_origin.x;
+            a.next.oy[nslot] = out.n_origin.y;
+            a.next.oz[nslot] = out.n_origin.z;
+            a.next.dx[nslot] = out.n_dir.x;
+            a.next.dy[nslot] = out.n_dir.y;
+            a.next.dz[nslot] = out.n_dir.z;
+            a.next.wx[nslot] = out.n_weight.x;
+            a.next.wy[nslot] = out.n_weight.y;
+            a.next.wz[nslot] = out.n_weight.z;
+            a.next.pdf[nslot] = out.n_pdf;
+            a.next.seed[nslot] = out.seed;
+            a.next.meta[nslot] = out.n_meta;
+            a.next.pixel[nslot] = out.pixel;
+        }
+    }
+}
+
+// 64 world-shader samples per texel; writes the finest mip level in 2x2-quad
+// order and the unnormalised pdf (device/kernels/envmap/build.metal:4-51).
+extern "C" __global__ __launch_bounds__(256) void rm_envmap_build_kernel(EnvmapArgs a) {
+    const int res = a.resolution;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (uint32_t)(res * res)) return;
+    const uint32_t x = tid % (uint32_t)res, y = tid / (uint32_t)res;
+    const uint32_t ray_index = y * (uint32_t)res + x;
+    float value = 0.0f;
+    for (int s = 0; s < 64; s++) {
+        Prng prng((uint32_t)s, ray_index);
+        const vec2 j = prng.sample2d();
+        const vec2 projected((( float)x + j.x) / (float)res, ((float)y + j.y) / (float)res);
+        const vec3 wo = warp::uniform_square_to_sphere(projected);
+        ShadingContext shading;
+        shading.ray_flags = 0;
+        shading.rnd = prng.sample3d();
+        shading.wo = -wo;
+        evaluate_environment(a.ctx, shading);
+        value += mean(shading.material.emission);
+    }
+    value /= 64.0f;
+    value += 1e-8f;
+    const uint32_t quad = (y / 2u) * ((uint32_t)res / 2u) + (x / 2u);
+    a.mipmap_last[4u * quad + (x & 1u) + 2u * (y & 1u)] = value;
+    a.pdfs[y * (uint32_t)res + x] = value;
+}
+
+}  // namespace rm

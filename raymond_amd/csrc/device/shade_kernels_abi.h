@@ -1,0 +1,29 @@
+// Argument blocks of the run-time compiled kernels (shade_kernels.h), shared
+// with the host code that launches them.
+#pragma once
+#include "common.h"
+#include "queues.h"
+#include "scene.h"
+
+namespace rm {
+
+struct ShadeArgs {
+    SceneView ctx;
+    PathQueue cur;
+    HitQueue hits;
+    PathQueue next;
+    ShadowQueue shadows;
+    Counters* counters;
+    float* image;          // RGBA accumulation
+    FrameUniforms u;
+    uint32_t depth;
+};
+
+struct EnvmapArgs {
+    SceneView ctx;
+    float* mipmap_last;    // finest mip level (quad layout)
+    float* pdfs;
+    int resolution;
+};
+
+}  // namespace rm

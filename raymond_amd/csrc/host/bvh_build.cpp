@@ -1,0 +1,362 @@
+// Host BVH8 builder: binned-SAH binary BVH, greedy collapse to 8 children by
+// largest surface area, octant slot assignment and conservative 8-bit
+// quantisation (layout in device/bvh.h).
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace rm {
+namespace {
+
+struct Aabb {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Aabb& b) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    void grow(const float p[3]) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    double area() const {
+        const double dx = std::max(0.0, (double)hi[0] - lo[0]);
+        const double dy = std::max(0.0, (double)hi[1] - lo[1]);
+        const double dz = std::max(0.0, (double)hi[2] - lo[2]);
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+    bool valid() const { return lo[0] <= hi[0]; }
+};
+
+struct Node2 {
+    Aabb box;
+    int32_t left = -1, right = -1;
+    uint32_t first = 0, count = 0;
+};
+
+constexpr int kBins = 32;
+constexpr int kMaxLeaf = 3;
+
+struct Builder2 {
+    const std::vector<Aabb>& tb;
+    const std::vector<std::array<float, 3>>& cent;
+    std::vector<uint32_t>& idx;
+    std::vector<Node2> nodes;
+    std::atomic<int> next{0};
+    std::atomic<int> threads_left{0};
+
+    Builder2(const std::vector<Aabb>& tb_, const std::vector<std::array<float, 3>>& c, std::vector<uint32_t>& i)
+        : tb(tb_), cent(c), idx(i) {}
+
+    int alloc(int n) { return next.fetch_add(n); }
+
+    void make_leaf(int ni, uint32_t begin, uint32_t end) {
+        nodes[ni].left = nodes[ni].right = -1;
+        nodes[ni].first = begin;
+        nodes[ni].count = end - begin;
+    }
+
+    void build(int ni, uint32_t begin, uint32_t end) {
+        Aabb box, cbox;
+        for (uint32_t i = begin; i < end; i++) {
+            box.grow(tb[idx[i]]);
+            cbox.grow(cent[idx[i]].data());
+        }
+        nodes[ni].box = box;
+        const uint32_t n = end - begin;
+        if (n == 1) {
+            make_leaf(ni, begin, end);
+            return;
+        }
+        // binned SAH over the three axes
+        double best_cost = INFINITY;
+        int best_axis = -1, best_split = -1;
+        const double parent_area = box.area();
+        for (int a = 0; a < 3; a++) {
+            const float ext = cbox.hi[a] - cbox.lo[a];
+            if (!(ext > 0.0f)) continue;
+            Aabb bb[kBins];
+            uint32_t bc[kBins] = {0};
+            const float k = (float)kBins * (1.0f - 1e-6f) / ext;
+            for (uint32_t i = begin; i < end; i++) {
+                int b = (int)((cent[idx[i]][a] - cbox.lo[a]) * k);
+                b = std::min(std::max(b, 0), kBins - 1);
+                bc[b]++;
+                bb[b].grow(tb[idx[i]]);
+            }
+            double ra[kBins];
+            uint32_t rc[kBins];
+            Aabb acc;
+            uint32_t cnt = 0;
+            for (int b = kBins - 1; b > 0; b--) {
+                acc.grow(bb[b]);
+                cnt += bc[b];
+                ra[b] = acc.valid() ? acc.area() : 0.0;
+                rc[b] = cnt;
+            }
+            Aabb lacc;
+            uint32_t lcnt = 0;
+            for (int b = 0; b < kBins - 1; b++) {
+                lacc.grow(bb[b]);
+                lcnt += bc[b];
+                if (lcnt == 0 || rc[b + 1] == 0) continue;
+                const double cost = 1.0 + (lacc.area() * lcnt + ra[b + 1] * rc[b + 1]) / std::max(parent_area, 1e-30);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_split = b;
+                }
+            }
+        }
+        if (n <= (uint32_t)kMaxLeaf && (best_axis < 0 || (double)n <= best_cost)) {
+            make_leaf(ni, begin, end);
+            return;
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = begin + n / 2;  // coincident centroids: split the range
+        } else {
+            const int a = best_axis;
+            const float ext = cbox.hi[a] - cbox.lo[a];
+            const float k = (float)kBins * (1.0f - 1e-6f) / ext;
+            auto it = std::partition(idx.begin() + begin, idx.begin() + end, [&](uint32_t t) {
+                int b = (int)((cent[t][a] - cbox.lo[a]) * k);
+                b = std::min(std::max(b, 0), kBins - 1);
+                return b <= best_split;
+            });
+            mid = (uint32_t)(it - idx.begin());
+            if (mid == begin || mid == end) mid = begin + n / 2;
+        }
+        const int l = alloc(2);
+        nodes[ni].left = l;
+        nodes[ni].right = l + 1;
+        if (n > 65536 && threads_left.fetch_sub(1) > 0) {
+            std::thread th([&, l, begin, mid] { build(l, begin, mid); });
+            build(l + 1, mid, end);
+            th.join();
+        } else {
+            build(l, begin, mid);
+            build(l + 1, mid, end);
+        }
+    }
+};
+
+// Quantisation of one node; returns false if a child box cannot be covered.
+bool quantise(Bvh8Node& out, const Aabb& node_box, const Aabb* child, const bool* used) {
+    out.px = node_box.lo[0];
+    out.py = node_box.lo[1];
+    out.pz = node_box.lo[2];
+    const float p[3] = {out.px, out.py, out.pz};
+    uint8_t e[3];
+    for (int a = 0; a < 3; a++) {
+        const double ext = (double)node_box.hi[a] - (double)node_box.lo[a];
+        int ee = 1;
+        if (ext > 0.0) {
+            ee = (int)std::ceil(std::log2(ext / 255.0)) + 127;
+            ee = std::max(1, std::min(254, ee));
+        }
+        for (;; ee++) {
+            if (ee > 254) return false;
+            const float scale = exp_scale((uint8_t)ee);
+            bool ok = true;
+            for (int c = 0; c < 8 && ok; c++) {
+                if (!used[c]) continue;
+                const double rlo = ((double)child[c].lo[a] - p[a]) / scale;
+                const double rhi = ((double)child[c].hi[a] - p[a]) / scale;
+                int qlo = (int)std::floor(rlo), qhi = (int)std::ceil(rhi);
+                qlo = std::max(0, std::min(255, qlo));
+                qhi = std::max(0, std::min(255, qhi));
+                while (qlo > 0 && decode_q(p[a], (uint8_t)qlo, scale) > child[c].lo[a]) qlo--;
+                while (qhi < 255 && decode_q(p[a], (uint8_t)qhi, scale) < child[c].hi[a]) qhi++;
+                if (decode_q(p[a], (uint8_t)qlo, scale) > child[c].lo[a] ||
+                    decode_q(p[a], (uint8_t)qhi, scale) < child[c].hi[a]) {
+                    ok = false;
+                    break;
+                }
+                uint8_t* lo_arr = a == 0 ? out.qlo_x : (a == 1 ? out.qlo_y : out.qlo_z);
+                uint8_t* hi_arr = a == 0 ? out.qhi_x : (a == 1 ? out.qhi_y : out.qhi_z);
+                lo_arr[c] = (uint8_t)qlo;
+                hi_arr[c] = (uint8_t)qhi;
+            }
+            if (ok) break;
+        }
+        e[a] = (uint8_t)ee;
+    }
+    out.ex = e[0];
+    out.ey = e[1];
+    out.ez = e[2];
+    return true;
+}
+
+}  // namespace
+
+Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error) {
+    const auto t0 = std::chrono::steady_clock::now();
+    Bvh8Build res;
+    const size_t n = tris.size();
+    if (n == 0) {
+        // an empty tree: root with no children, every ray misses
+        Bvh8Node root;
+        std::memset(&root, 0, sizeof(root));
+        res.nodes.push_back(root);
+        res.depth = 1;
+        return res;
+    }
+    if (n >= (1u << 31)) {
+        if (error) *error = "too many triangles";
+        return res;
+    }
+    std::vector<Aabb> tb(n);
+    std::vector<std::array<float, 3>> cent(n);
+    for (size_t i = 0; i < n; i++) {
+        const BuildTri& t = tris[i];
+        const float* vs[3] = {&t.v0.x, &t.v1.x, &t.v2.x};
+        Aabb b;
+        for (int v = 0; v < 3; v++) b.grow(vs[v]);
+        tb[i] = b;
+        for (int a = 0; a < 3; a++) cent[i][a] = 0.5f * (b.lo[a] + b.hi[a]);
+    }
+    std::vector<uint32_t> idx(n);
+    for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+
+    Builder2 b2(tb, cent, idx);
+    b2.nodes.resize(2 * n + 1);
+    int hw = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    b2.threads_left = std::max(0, std::min(hw, 64) - 1);
+    b2.next = 1;
+    b2.build(0, 0, (uint32_t)n);
+    res.bvh2_nodes = b2.next.load();
+    const std::vector<Node2>& N2 = b2.nodes;
+
+    // Collapse: each BVH8 node lists up to 8 BVH2 nodes as children.
+    struct Pending {
+        int n2;      // BVH2 node this BVH8 node expands
+        int index;   // BVH8 node index
+        int depth;
+    };
+    std::vector<Pending> queue;
+    res.nodes.reserve(n / 2 + 8);
+    res.nodes.emplace_back();
+    // A leaf root is wrapped: the BVH8 root gets it as its only (leaf) child.
+    queue.push_back({0, 0, 1});
+    size_t head = 0;
+    int max_depth = 1;
+    while (head < queue.size()) {
+        const Pending pd = queue[head++];
+        max_depth = std::max(max_depth, pd.depth);
+        int ch[8];
+        int nc = 0;
+        const Node2& root2 = N2[pd.n2];
+        if (root2.left < 0) {
+            ch[nc++] = pd.n2;
+        } else {
+            ch[nc++] = root2.left;
+            ch[nc++] = root2.right;
+            while (nc < 8) {
+                int best = -1;
+                double best_area = -1.0;
+                for (int c = 0; c < nc; c++) {
+                    if (N2[ch[c]].left < 0) continue;
+                    const double a = N2[ch[c]].box.area();
+                    if (a > best_area) {
+                        best_area = a;
+                        best = c;
+                    }
+                }
+                if (best < 0) break;
+                const Node2& e = N2[ch[best]];
+                ch[best] = e.left;
+                ch[nc++] = e.right;
+            }
+        }
+        // octant slot assignment: slot k takes the child nearest corner k
+        Aabb node_box;
+        for (int c = 0; c < nc; c++) node_box.grow(N2[ch[c]].box);
+        double pc[3], cc[8][3];
+        for (int a = 0; a < 3; a++) pc[a] = 0.5 * ((double)node_box.lo[a] + node_box.hi[a]);
+        for (int c = 0; c < nc; c++)
+            for (int a = 0; a < 3; a++) cc[c][a] = 0.5 * ((double)N2[ch[c]].box.lo[a] + N2[ch[c]].box.hi[a]) - pc[a];
+        struct Cost { double v; int c, s; };
+        std::vector<Cost> costs;
+        for (int c = 0; c < nc; c++)
+            for (int s = 0; s < 8; s++) {
+                double v = 0;
+                for (int a = 0; a < 3; a++) v -= cc[c][a] * (((s >> a) & 1) ? 1.0 : -1.0);
+                costs.push_back({v, c, s});
+            }
+        std::sort(costs.begin(), costs.end(), [](const Cost& x, const Cost& y) {
+            if (x.v != y.v) return x.v < y.v;
+            if (x.c != y.c) return x.c < y.c;
+            return x.s < y.s;
+        });
+        int slot_child[8];
+        bool child_done[8] = {false}, slot_used[8] = {false};
+        for (int s = 0; s < 8; s++) slot_child[s] = -1;
+        for (const Cost& k : costs) {
+            if (child_done[k.c] || slot_used[k.s]) continue;
+            child_done[k.c] = slot_used[k.s] = true;
+            slot_child[k.s] = k.c;
+        }
+        Bvh8Node out;
+        std::memset(&out, 0, sizeof(out));
+        Aabb cb[8];
+        bool used[8] = {false};
+        int n_internal = 0;
+        for (int s = 0; s < 8; s++) {
+            if (slot_child[s] < 0) continue;
+            used[s] = true;
+            cb[s] = N2[ch[slot_child[s]]].box;
+            if (N2[ch[slot_child[s]]].left >= 0) n_internal++;
+        }
+        if (!quantise(out, node_box, cb, used)) {
+            if (error) *error = "BVH8 quantisation failed";
+            return Bvh8Build();
+        }
+        out.child_base = (uint32_t)res.nodes.size();
+        out.tri_base = (uint32_t)res.tris.size();
+        uint32_t tri_off = 0;
+        int rank = 0;
+        for (int s = 0; s < 8; s++) {
+            if (slot_child[s] < 0) continue;
+            const Node2& c2 = N2[ch[slot_child[s]]];
+            if (c2.left >= 0) {
+                out.imask |= (uint8_t)(1u << s);
+                out.meta[s] = (uint8_t)((1u << 5) | (24u + s));
+                queue.push_back({ch[slot_child[s]], (int)out.child_base + rank, pd.depth + 1});
+                rank++;
+            } else {
+                const uint32_t cnt = c2.count;
+                out.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | tri_off);
+                for (uint32_t t = 0; t < cnt; t++) {
+                    const BuildTri& bt = tris[idx[c2.first + t]];
+                    BvhTri r;
+                    r.v0x = bt.v0.x; r.v0y = bt.v0.y; r.v0z = bt.v0.z; r.inst = bt.inst;
+                    r.v1x = bt.v1.x; r.v1y = bt.v1.y; r.v1z = bt.v1.z; r.prim = bt.prim;
+                    r.v2x = bt.v2.x; r.v2y = bt.v2.y; r.v2z = bt.v2.z; r.pad = 0;
+                    res.tris.push_back(r);
+                }
+                tri_off += cnt;
+            }
+        }
+        for (int r = 0; r < n_internal; r++) res.nodes.emplace_back();
+        res.nodes[pd.index] = out;
+    }
+    res.depth = max_depth;
+    if (max_depth >= 60) {
+        if (error) *error = "BVH8 too deep for the traversal stack";
+        return Bvh8Build();
+    }
+    res.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return res;
+}
+
+}  // namespace rm

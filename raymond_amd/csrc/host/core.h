@@ -1,0 +1,129 @@
+// Internal definitions of the C-ABI objects (include/rm.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/rm.h"
+#include "../device/bvh.h"
+#include "../device/queues.h"
+#include "../device/scene.h"
+
+namespace rm {
+
+void set_error(const std::string& msg);
+
+struct HipError {
+    int code;
+    std::string msg;
+};
+
+#define RM_HIP(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            throw ::rm::HipError{RM_ERROR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+#define RM_REQUIRE(cond, msg)                                        \
+    do {                                                             \
+        if (!(cond)) throw ::rm::HipError{RM_ERROR_INVALID, (msg)}; \
+    } while (0)
+
+// Device allocation owned by a C-ABI object.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        RM_HIP(hipMalloc(&ptr, n));
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+    void upload(const void* src, size_t n, hipStream_t s) {
+        alloc(n);
+        if (n) RM_HIP(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s));
+    }
+};
+
+struct JitModule {
+    hipModule_t module = nullptr;
+    hipFunction_t shade = nullptr;
+    hipFunction_t envmap_build = nullptr;
+    ~JitModule() {
+        if (module) (void)hipModuleUnload(module);
+    }
+};
+
+// Compiles material source + the shading kernels; caches code objects on disk.
+bool jit_compile(const std::string& body, const std::string& options, std::vector<char>& code, std::string& log,
+                 bool& cache_hit);
+// Full translation unit handed to hiprtc (exposed for inspection/tests).
+std::string jit_translation_unit(const std::string& body);
+
+}  // namespace rm
+
+struct rm_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    int trace_blocks_per_cu = 1;
+    int any_blocks_per_cu = 1;
+};
+
+struct rm_scene {
+    rm_device* dev = nullptr;
+    // host copies
+    std::vector<float> vertices, normals, texcoords;
+    std::vector<uint32_t> indices;
+    std::vector<uint16_t> materials;
+    std::vector<rm_instance_desc> inst_desc;
+    std::vector<rm::PerInstance> instances;
+    std::vector<rm::AreaLight> area;
+    std::vector<rm::PointLight> point;
+    std::vector<rm::SunLight> sun;
+    std::vector<rm::SpotLight> spot;
+    std::vector<rm::ShapeLight> shape;
+    std::vector<float> light_faces;
+    uint32_t world_shader = 0;
+    int envmap_res = 2048;
+    // device copies
+    rm::DevBuf d_vertices, d_normals, d_texcoords, d_indices, d_materials, d_instances;
+    rm::DevBuf d_area, d_point, d_sun, d_spot, d_shape, d_light_faces;
+    rm::DevBuf d_nodes, d_tris;
+    rm::DevBuf d_env_pdfs, d_env_mip;
+    rm::DevBuf d_spill;
+    size_t spill_threads = 0;
+    rm::SceneView view{};
+    bool accel_built = false;
+    bool materials_built = false;
+    bool envmap_built = false;
+    rm::JitModule jit;
+    rm_scene_stats stats{};
+};
+
+struct rm_renderer {
+    rm_scene* scene = nullptr;
+    uint32_t width = 0, height = 0, max_depth = 8, spl = 1;
+    size_t capacity = 0;
+    rm::DevBuf q_buf[2], hit_buf, shadow_buf, counters_buf, image_buf, image64_buf, resolve_buf;
+    rm::PathQueue q[2]{};
+    rm::HitQueue hits{};
+    rm::ShadowQueue shadows{};
+    rm_uniforms u{};
+    uint32_t frame_index = 0;  // samples accumulated so far
+    std::vector<hipEvent_t> events;
+};

@@ -1,0 +1,282 @@
+// The bounce loop (host/Renderer.swift:480-696) as a sequence of persistent
+// kernels on one HIP stream.  Queue lengths stay on the device; the host only
+// enqueues launches and reads the per-depth counters once per launch.
+#include <chrono>
+#include <cstring>
+
+#include "../device/shade_kernels_abi.h"
+#include "../kernels/misc_api.h"
+#include "../kernels/trace_api.h"
+#include "core.h"
+#include "scene_api.h"
+
+namespace rm {
+
+static void carve_path_queue(DevBuf& buf, size_t cap, PathQueue& q) {
+    buf.alloc(cap * 13 * 4);
+    float* f = buf.as<float>();
+    float** fields[10] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.wx, &q.wy, &q.wz, &q.pdf};
+    for (int k = 0; k < 10; k++) *fields[k] = f + (size_t)k * cap;
+    uint32_t* u = reinterpret_cast<uint32_t*>(f + 10 * cap);
+    q.seed = u;
+    q.meta = u + cap;
+    q.pixel = u + 2 * cap;
+}
+
+void renderer_create(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, uint32_t spl, rm_renderer* r) {
+    RM_REQUIRE(w > 0 && h > 0, "image size must be positive");
+    RM_REQUIRE(max_depth >= 1 && max_depth <= (uint32_t)kMaxDepth, "max_depth out of range");
+    RM_REQUIRE(spl >= 1, "samples_per_launch must be >= 1");
+    RM_REQUIRE((uint64_t)w * h * spl < (1ull << 31), "wavefront too large");
+    r->scene = s;
+    r->width = w;
+    r->height = h;
+    r->max_depth = max_depth;
+    r->spl = spl;
+    r->capacity = (size_t)w * h * spl;
+    const size_t cap = r->capacity;
+    carve_path_queue(r->q_buf[0], cap, r->q[0]);
+    carve_path_queue(r->q_buf[1], cap, r->q[1]);
+    r->hit_buf.alloc(cap * 5 * 4);
+    float* hf = r->hit_buf.as<float>();
+    r->hits.t = hf;
+    r->hits.prim = reinterpret_cast<uint32_t*>(hf + cap);
+    r->hits.inst = reinterpret_cast<uint32_t*>(hf + 2 * cap);
+    r->hits.b0 = hf + 3 * cap;
+    r->hits.b1 = hf + 4 * cap;
+    r->shadow_buf.alloc(cap * 11 * 4);
+    float* sf = r->shadow_buf.as<float>();
+    ShadowQueue& sq = r->shadows;
+    float** sfields[10] = {&sq.ox, &sq.oy, &sq.oz, &sq.dx, &sq.dy, &sq.dz, &sq.tmax, &sq.wx, &sq.wy, &sq.wz};
+    for (int k = 0; k < 10; k++) *sfields[k] = sf + (size_t)k * cap;
+    sq.pixel = reinterpret_cast<uint32_t*>(sf + 10 * cap);
+    r->counters_buf.alloc(sizeof(Counters));
+    r->image_buf.alloc((size_t)w * h * 16);
+    r->resolve_buf.alloc((size_t)w * h * 16);
+    r->u.sampling_mode = RM_SAMPLING_MIS;
+    r->u.accumulate = 1;
+    r->u.exposure = 1.0f;
+    r->u.tonemapping = RM_TONEMAP_LINEAR;
+    r->u.rr_depth = 0;
+    renderer_reset(r);
+}
+
+void renderer_reset(rm_renderer* r) {
+    hipStream_t st = r->scene->dev->stream;
+    RM_HIP(hipMemsetAsync(r->image_buf.ptr, 0, r->image_buf.bytes, st));
+    if (r->image64_buf.ptr) RM_HIP(hipMemsetAsync(r->image64_buf.ptr, 0, r->image64_buf.bytes, st));
+    RM_HIP(hipStreamSynchronize(st));
+    r->frame_index = 0;
+}
+
+namespace {
+
+struct StageTimer {
+    bool on;
+    hipStream_t st;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, shade, any;
+    std::vector<hipEvent_t>& pool;
+    size_t used = 0;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            RM_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    hipEvent_t begin() {
+        if (!on) return nullptr;
+        hipEvent_t e = get();
+        RM_HIP(hipEventRecord(e, st));
+        return e;
+    }
+    void end(hipEvent_t b, std::vector<std::pair<hipEvent_t, hipEvent_t>>& into) {
+        if (!on) return;
+        hipEvent_t e = get();
+        RM_HIP(hipEventRecord(e, st));
+        into.push_back({b, e});
+    }
+    static double sum(const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) {
+        double ms = 0;
+        for (auto& p : v) {
+            float t = 0;
+            RM_HIP(hipEventElapsedTime(&t, p.first, p.second));
+            ms += t;
+        }
+        return ms;
+    }
+};
+
+}  // namespace
+
+static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
+    rm_scene* s = r->scene;
+    rm_device* dev = s->dev;
+    hipStream_t st = dev->stream;
+    const uint32_t W = r->width, H = r->height;
+    const bool tstats = r->u.traversal_stats != 0;
+    Counters* cnt = r->counters_buf.as<Counters>();
+    float* image = r->image_buf.as<float>();
+    const int tblocks = dev->props.multiProcessorCount * dev->trace_blocks_per_cu;
+    const int ablocks = dev->props.multiProcessorCount * dev->any_blocks_per_cu;
+    int shade_bpc = 0;
+    RM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&shade_bpc, s->jit.shade, 256, 0));
+    if (shade_bpc < 1) shade_bpc = 1;
+    const int sblocks = dev->props.multiProcessorCount * shade_bpc;
+
+    StageTimer tm{r->u.timing != 0, st, {}, {}, {}, r->events};
+
+    if (!r->u.accumulate) RM_HIP(hipMemsetAsync(image, 0, r->image_buf.bytes, st));
+    RM_HIP(launch_clear_counters(cnt, W * H * n, st));
+
+    FrameUniforms fu{};
+    fu.width = W;
+    fu.height = H;
+    fu.samples = n;
+    fu.random_seed = r->frame_index + 1;
+    fu.sampling_mode = r->u.sampling_mode;
+    fu.rr_depth = r->u.rr_depth;
+    fu.max_depth = r->max_depth;
+    fu.image_offset = 0;
+
+    ClosestArgs ca{};
+    ca.nodes = s->d_nodes.as<Bvh8Node>();
+    ca.tris = s->d_tris.as<BvhTri>();
+    ca.hits = r->hits;
+    ca.spill = s->d_spill.as<uint2>();
+    ca.counters = cnt;
+    ca.camera = s->view.camera;
+    ca.width = W;
+    ca.height = H;
+    ca.random_seed = fu.random_seed;
+    ca.image_offset = 0;
+
+    // depth 0: camera rays generated and traced in one kernel (generate.metal:72-192)
+    ca.mode = kTraceCamera;
+    ca.count = W * H * n;
+    ca.count_ptr = nullptr;
+    ca.fetch = &cnt->fetch_rays[0];
+    ca.paths = r->q[0];
+    hipEvent_t e0 = tm.begin();
+    RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
+    tm.end(e0, tm.closest);
+
+    for (uint32_t depth = 0; depth < r->max_depth; depth++) {
+        PathQueue& cur = r->q[depth & 1];
+        PathQueue& next = r->q[(depth + 1) & 1];
+        if (depth > 0) {
+            ca.mode = kTraceQueue;
+            ca.count_ptr = &cnt->rays[depth];
+            ca.fetch = &cnt->fetch_rays[depth];
+            ca.paths = cur;
+            hipEvent_t e = tm.begin();
+            RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
+            tm.end(e, tm.closest);
+        }
+        ShadeArgs sa;
+        sa.ctx = s->view;
+        sa.cur = cur;
+        sa.hits = r->hits;
+        sa.next = next;
+        sa.shadows = r->shadows;
+        sa.counters = cnt;
+        sa.image = image;
+        sa.u = fu;
+        sa.depth = depth;
+        void* args[] = {&sa};
+        hipEvent_t es = tm.begin();
+        RM_HIP(hipModuleLaunchKernel(s->jit.shade, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
+        tm.end(es, tm.shade);
+        // shadow rays of the last depth are never traced (Renderer.swift:623-625)
+        if (depth + 1 == r->max_depth) break;
+        AnyArgs aa{};
+        aa.nodes = ca.nodes;
+        aa.tris = ca.tris;
+        aa.count_ptr = &cnt->shadows[depth];
+        aa.fetch = &cnt->fetch_shadows[depth];
+        aa.shadows = r->shadows;
+        aa.image = image;
+        aa.spill = ca.spill;
+        aa.counters = cnt;
+        hipEvent_t ea = tm.begin();
+        RM_HIP(launch_trace_any(aa, ablocks, tstats, st));
+        tm.end(ea, tm.any);
+    }
+    if (r->u.accumulate_f64) {
+        if (!r->image64_buf.ptr) {
+            r->image64_buf.alloc((size_t)W * H * 4 * sizeof(double));
+            RM_HIP(hipMemsetAsync(r->image64_buf.ptr, 0, r->image64_buf.bytes, st));
+        }
+        RM_HIP(launch_accum_f64(image, r->image64_buf.as<double>(), W * H * 4, st));
+    }
+    Counters host{};
+    RM_HIP(hipMemcpyAsync(&host, cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+    r->frame_index += n;
+    if (stats) {
+        for (uint32_t d = 0; d < r->max_depth; d++) {
+            stats->rays[d] += host.rays[d];
+            stats->total_rays += host.rays[d];
+            if (d + 1 < r->max_depth) {
+                stats->shadow_rays[d] += host.shadows[d];
+                stats->total_rays += host.shadows[d];
+            }
+        }
+        stats->samples += (uint64_t)W * H * n;
+        stats->trace_closest_launches += (uint32_t)tm.closest.size();
+        stats->trace_any_launches += (uint32_t)tm.any.size();
+        if (tm.on) {
+            stats->ms_trace_closest += StageTimer::sum(tm.closest);
+            stats->ms_shade += StageTimer::sum(tm.shade);
+            stats->ms_trace_any += StageTimer::sum(tm.any);
+        }
+        stats->nodes_closest += host.nodes_closest;
+        stats->tris_closest += host.tris_closest;
+        stats->nodes_any += host.nodes_any;
+        stats->tris_any += host.tris_any;
+    }
+}
+
+void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
+    rm_scene* s = r->scene;
+    RM_REQUIRE(s->accel_built, "acceleration structure not built");
+    RM_REQUIRE(s->materials_built, "materials not compiled");
+    RM_REQUIRE(s->envmap_built, "envmap not built");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t left = samples;
+    while (left > 0) {
+        const uint32_t n = left < r->spl ? left : r->spl;
+        render_launch(r, n, stats);
+        left -= n;
+    }
+    if (stats) stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void renderer_read_image(rm_renderer* r, float* rgba, size_t count) {
+    RM_REQUIRE(count >= (size_t)r->width * r->height * 4, "output buffer too small");
+    hipStream_t st = r->scene->dev->stream;
+    RM_HIP(hipMemcpyAsync(rgba, r->image_buf.ptr, (size_t)r->width * r->height * 16, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+}
+
+void renderer_read_image_f64(rm_renderer* r, double* rgba, size_t count) {
+    RM_REQUIRE(count >= (size_t)r->width * r->height * 4, "output buffer too small");
+    RM_REQUIRE(r->image64_buf.ptr != nullptr, "FP64 accumulation not enabled");
+    hipStream_t st = r->scene->dev->stream;
+    RM_HIP(hipMemcpyAsync(rgba, r->image64_buf.ptr, (size_t)r->width * r->height * 32, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+}
+
+void renderer_resolve(rm_renderer* r, float* rgba, size_t count) {
+    RM_REQUIRE(count >= (size_t)r->width * r->height * 4, "output buffer too small");
+    hipStream_t st = r->scene->dev->stream;
+    const float divisor = r->u.accumulate ? (float)(r->frame_index > 0 ? r->frame_index : 1) : 1.0f;
+    RM_HIP(launch_resolve(r->image_buf.as<float>(), r->resolve_buf.as<float>(), r->width * r->height, r->u.exposure,
+                          divisor, (int)r->u.tonemapping, st));
+    RM_HIP(hipMemcpyAsync(rgba, r->resolve_buf.ptr, (size_t)r->width * r->height * 16, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace rm

@@ -1,0 +1,31 @@
+// Internal entry points behind the C ABI (capi.cpp).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "core.h"
+
+namespace rm {
+
+mat3 normal_transform_of(const mat4& m);
+float build_light_distribution(const mat3& M, const uint32_t* idx, const float* verts, uint32_t face_count, float* out);
+std::vector<BuildTri> world_triangles(const rm_scene* s);
+
+void scene_create(rm_device* dev, const rm_scene_desc* d, rm_scene* s);
+void scene_set_camera(rm_scene* s, const rm_camera_desc* c);
+void scene_build_accel(rm_scene* s, int threads);
+void scene_compile_materials(rm_scene* s, const char* src, size_t len, const char* options, std::string& log);
+void scene_build_envmap(rm_scene* s);
+void trace_closest_raw(rm_scene* s, const float* rays, uint32_t n, float* t, uint32_t* inst, uint32_t* prim,
+                       float* bary);
+void trace_any_raw(rm_scene* s, const float* rays, uint32_t n, uint32_t* occluded);
+
+void renderer_create(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, uint32_t spl, rm_renderer* r);
+void renderer_reset(rm_renderer* r);
+void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
+void renderer_read_image(rm_renderer* r, float* rgba, size_t count);
+void renderer_read_image_f64(rm_renderer* r, double* rgba, size_t count);
+void renderer_resolve(rm_renderer* r, float* rgba, size_t count);
+
+}  // namespace rm

@@ -1,0 +1,51 @@
+// Launch interface of the ahead-of-time traversal kernels (trace.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../device/bvh.h"
+#include "../device/queues.h"
+#include "../device/scene.h"
+
+namespace rm {
+
+constexpr int kTraceBlock = 64;      // one wave64 per workgroup
+constexpr int kMaxTraceStack = 64;   // LDS rows + spill rows; BVH depth is bounded below this
+constexpr int kTraceCamera = 0;
+constexpr int kTraceQueue = 1;
+
+struct ClosestArgs {
+    const Bvh8Node* nodes;
+    const BvhTri* tris;
+    int mode;
+    uint32_t count;             // used when count_ptr is null
+    const uint32_t* count_ptr;  // device queue length
+    uint32_t* fetch;            // persistent work counter (zero before launch)
+    PathQueue paths;
+    HitQueue hits;
+    const float* tmin_arr;      // optional per-ray range (raw batches); else eps / inf
+    const float* tmax_arr;
+    uint2* spill;               // (kMaxTraceStack - 16) x grid threads
+    Counters* counters;
+    // camera mode
+    Camera camera;
+    uint32_t width, height, random_seed, image_offset;
+};
+
+struct AnyArgs {
+    const Bvh8Node* nodes;
+    const BvhTri* tris;
+    uint32_t count;
+    const uint32_t* count_ptr;
+    uint32_t* fetch;
+    ShadowQueue shadows;
+    const float* tmin_arr;      // optional; else eps
+    float* image;               // accumulate unoccluded weights when non-null
+    uint32_t* occluded;         // else write 0/1 per ray
+    uint2* spill;
+    Counters* counters;
+};
+
+hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream);
+hipError_t launch_trace_any(const AnyArgs& a, int blocks, bool stats, hipStream_t stream);
+int trace_blocks_per_cu(bool any);
+
+}  // namespace rm

@@ -1,0 +1,462 @@
+"""Synthetic scenes for the benchmark configurations, as scene descriptions for
+the C ABI (include/rm.h) plus node-graph materials in the Blender exporter's
+JSON schema (scripts/blender_exporter/raymond_blender/materials.py:300-560).
+
+Entity, material and light order is fixed by construction (the reference's
+order follows Swift dictionary iteration, EntityBuilder.swift:50-52,85, and
+is not reproducible).
+"""
+import ctypes as C
+import json
+import math
+
+import numpy as np
+
+from . import api
+
+# ---------------------------------------------------------------------------
+# Node-graph helpers (exporter schema)
+
+
+def _in(t, value=None, link=None):
+    d = {"type": t}
+    if value is not None:
+        d["value"] = value
+    if link is not None:
+        d["links"] = [[link[0], link[1]]]
+    return d
+
+
+PRINCIPLED_DEFAULTS = {
+    "Base Color": ("RGBA", [0.8, 0.8, 0.8, 1.0]), "Subsurface": ("VALUE", 0.0),
+    "Subsurface Radius": ("VECTOR", [1.0, 0.2, 0.1]), "Subsurface Color": ("RGBA", [0.8, 0.8, 0.8, 1.0]),
+    "Subsurface IOR": ("VALUE", 1.4), "Subsurface Anisotropy": ("VALUE", 0.0), "Metallic": ("VALUE", 0.0),
+    "Specular": ("VALUE", 0.5), "Specular Tint": ("VALUE", 0.0), "Roughness": ("VALUE", 0.5),
+    "Anisotropic": ("VALUE", 0.0), "Anisotropic Rotation": ("VALUE", 0.0), "Sheen": ("VALUE", 0.0),
+    "Sheen Tint": ("VALUE", 0.5), "Clearcoat": ("VALUE", 0.0), "Clearcoat Roughness": ("VALUE", 0.03),
+    "IOR": ("VALUE", 1.45), "Transmission": ("VALUE", 0.0), "Transmission Roughness": ("VALUE", 0.0),
+    "Emission": ("RGBA", [0.0, 0.0, 0.0, 1.0]), "Emission Strength": ("VALUE", 1.0), "Alpha": ("VALUE", 1.0),
+    "Normal": ("VECTOR", [0.0, 0.0, 0.0]), "Clearcoat Normal": ("VECTOR", [0.0, 0.0, 0.0]),
+    "Tangent": ("VECTOR", [0.0, 0.0, 0.0]), "Weight": ("VALUE", 0.0),
+}
+
+
+def principled_node(**overrides):
+    inputs = {}
+    for k, (t, v) in PRINCIPLED_DEFAULTS.items():
+        inputs[k] = _in(t, v)
+    for k, v in overrides.items():
+        name = k.replace("_", " ")
+        t = PRINCIPLED_DEFAULTS[name][0]
+        if isinstance(v, tuple) and v and v[0] == "link":
+            inputs[name] = _in(t, PRINCIPLED_DEFAULTS[name][1], link=(v[1], v[2]))
+        else:
+            inputs[name] = _in(t, v)
+    return {"type": "BSDF_PRINCIPLED", "parameters": {"distribution": "GGX", "subsurface_method": "RANDOM_WALK"},
+            "inputs": inputs}
+
+
+def output_material(link):
+    return {"type": "OUTPUT_MATERIAL", "parameters": {},
+            "inputs": {"Surface": _in("SHADER", link=link), "Volume": _in("SHADER"),
+                       "Displacement": _in("VECTOR", [0.0, 0.0, 0.0]), "Thickness": _in("VALUE", 0.0)}}
+
+
+def diffuse_material(name, rgb, roughness=0.0):
+    return {"name": name, "nodes": {
+        "Diffuse BSDF": {"type": "BSDF_DIFFUSE", "parameters": {},
+                         "inputs": {"Color": _in("RGBA", list(rgb) + [1.0]), "Roughness": _in("VALUE", roughness),
+                                    "Normal": _in("VECTOR", [0.0, 0.0, 0.0]), "Weight": _in("VALUE", 0.0)}},
+        "Material Output": output_material(("Diffuse BSDF", "BSDF")),
+    }}
+
+
+def principled_material(name, **kw):
+    return {"name": name, "nodes": {
+        "Principled BSDF": principled_node(**kw),
+        "Material Output": output_material(("Principled BSDF", "BSDF")),
+    }}
+
+
+def emission_material(name, rgb, strength):
+    return {"name": name, "nodes": {
+        "Emission": {"type": "EMISSION", "parameters": {},
+                     "inputs": {"Color": _in("RGBA", list(rgb) + [1.0]), "Strength": _in("VALUE", strength),
+                                "Weight": _in("VALUE", 0.0)}},
+        "Material Output": output_material(("Emission", "Emission")),
+    }}
+
+
+def background_world(name, rgb, strength=1.0):
+    return {"name": name, "nodes": {
+        "Background": {"type": "BACKGROUND", "parameters": {},
+                       "inputs": {"Color": _in("RGBA", list(rgb) + [1.0]), "Strength": _in("VALUE", strength),
+                                  "Weight": _in("VALUE", 0.0)}},
+        "World Output": {"type": "OUTPUT_WORLD", "parameters": {},
+                         "inputs": {"Surface": _in("SHADER", link=("Background", "Background")),
+                                    "Volume": _in("SHADER"), "Thickness": _in("VALUE", 0.0)}},
+    }}
+
+
+SKY_PARAMS = dict(sun_direction=[0.35, -0.45, 0.82], sun_strength=1000.0, sun_sharpness=2000.0,
+                  zenith=[0.15, 0.3, 0.8, 1.0], horizon=[0.7, 0.8, 0.95, 1.0], ground=[0.25, 0.22, 0.2, 1.0])
+
+
+def sky_world(name="sky", p=SKY_PARAMS):
+    return {"name": name, "nodes": {
+        "Sky": {"type": "TEX_SKY_ANALYTIC", "parameters": {},
+                "inputs": {"Vector": _in("VECTOR", [0.0, 0.0, 0.0]),
+                           "Sun Direction": _in("VECTOR", p["sun_direction"]),
+                           "Sun Strength": _in("VALUE", p["sun_strength"]),
+                           "Sun Sharpness": _in("VALUE", p["sun_sharpness"]),
+                           "Zenith Color": _in("RGBA", p["zenith"]), "Horizon Color": _in("RGBA", p["horizon"]),
+                           "Ground Color": _in("RGBA", p["ground"])}},
+        "Background": {"type": "BACKGROUND", "parameters": {},
+                       "inputs": {"Color": _in("RGBA", [0.0, 0.0, 0.0, 1.0], link=("Sky", "Color")),
+                                  "Strength": _in("VALUE", 1.0), "Weight": _in("VALUE", 0.0)}},
+        "World Output": {"type": "OUTPUT_WORLD", "parameters": {},
+                         "inputs": {"Surface": _in("SHADER", link=("Background", "Background")),
+                                    "Volume": _in("SHADER"), "Thickness": _in("VALUE", 0.0)}},
+    }}
+
+
+# ---------------------------------------------------------------------------
+# Scene description
+
+
+def translate(t):
+    m = np.eye(4, dtype=np.float64)
+    m[:3, 3] = t
+    return m
+
+
+def rotate_z(deg):
+    a = math.radians(deg)
+    m = np.eye(4)
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = math.cos(a), -math.sin(a), math.sin(a), math.cos(a)
+    return m
+
+
+def scale(s):
+    m = np.eye(4)
+    m[0, 0], m[1, 1], m[2, 2] = s
+    return m
+
+
+def look_at(eye, target, up=(0.0, 0.0, 1.0)):
+    """Camera-to-world matrix; the camera looks along its -z axis with +y up."""
+    eye, target, up = (np.asarray(v, np.float64) for v in (eye, target, up))
+    back = eye - target
+    back /= np.linalg.norm(back)
+    right = np.cross(up, back)
+    right /= np.linalg.norm(right)
+    upv = np.cross(back, right)
+    m = np.eye(4)
+    m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = right, upv, back, eye
+    return m
+
+
+def col_major(m):
+    return [float(x) for x in np.asarray(m, np.float32).T.reshape(-1)]
+
+
+class Shape:
+    """Triangle mesh in object space; faces index its own vertices."""
+
+    def __init__(self, vertices, normals, faces, materials, texcoords=None):
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self.faces = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        self.materials = np.ascontiguousarray(np.broadcast_to(np.asarray(materials, np.uint16),
+                                                              (self.faces.shape[0],)))
+        if texcoords is None:
+            texcoords = np.zeros((self.vertices.shape[0], 2), np.float32)
+        self.texcoords = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+
+
+class SceneDescription:
+    """Host-side scene: shapes, entities, lights, camera and materials."""
+
+    def __init__(self, name):
+        self.name = name
+        self.shapes = []        # Shape
+        self.entities = []      # (shape index, 4x4 transform, visibility, emissive)
+        self.area_lights, self.point_lights, self.sun_lights, self.spot_lights = [], [], [], []
+        self.surface_materials = []
+        self.light_materials = []
+        self.world_shader = 0
+        self.camera = None      # dict(transform, near, far, focal, shift)
+        self.envmap_resolution = 2048
+        self._keep = []
+
+    def add_shape(self, shape):
+        self.shapes.append(shape)
+        return len(self.shapes) - 1
+
+    def add_entity(self, shape, transform=None, visibility=api.RAY_ALL, emissive=False):
+        self.entities.append((shape, np.eye(4) if transform is None else transform, visibility, emissive))
+
+    def materials_json(self):
+        return json.dumps({"surface": self.surface_materials, "light": self.light_materials})
+
+    def material_source(self, noinline=False):
+        return api.codegen(self.materials_json(), noinline)
+
+    def arrays(self):
+        """Concatenated geometry and per-instance ranges."""
+        vo, fo = [], []
+        nv = nf = 0
+        for s in self.shapes:
+            vo.append(nv)
+            fo.append(nf)
+            nv += s.vertices.shape[0]
+            nf += s.faces.shape[0]
+        verts = np.concatenate([s.vertices for s in self.shapes]) if self.shapes else np.zeros((0, 3), np.float32)
+        norms = np.concatenate([s.normals for s in self.shapes]) if self.shapes else np.zeros((0, 3), np.float32)
+        uvs = np.concatenate([s.texcoords for s in self.shapes]) if self.shapes else np.zeros((0, 2), np.float32)
+        faces = np.concatenate([s.faces for s in self.shapes]) if self.shapes else np.zeros((0, 3), np.uint32)
+        mats = np.concatenate([s.materials for s in self.shapes]) if self.shapes else np.zeros(0, np.uint16)
+        inst = []
+        for (si, m, vis, em) in self.entities:
+            s = self.shapes[si]
+            inst.append(dict(vertex_offset=vo[si], vertex_count=s.vertices.shape[0], face_offset=fo[si],
+                             face_count=s.faces.shape[0], transform=col_major(m), visibility=vis,
+                             emissive=int(em)))
+        return (np.ascontiguousarray(verts, np.float32), np.ascontiguousarray(norms, np.float32),
+                np.ascontiguousarray(uvs, np.float32), np.ascontiguousarray(faces, np.uint32),
+                np.ascontiguousarray(mats, np.uint16), inst)
+
+    def camera_c(self):
+        c = self.camera
+        return api.CameraDesc((C.c_float * 16)(*col_major(c["transform"])), c.get("near", 1e-3),
+                              c.get("far", 1e4), c["focal"], (C.c_float * 2)(*c.get("shift", (0.0, 0.0))))
+
+    def to_c(self):
+        verts, norms, uvs, faces, mats, inst = self.arrays()
+        self._keep = [verts, norms, uvs, faces, mats]
+        sd = api.SceneDesc()
+        sd.num_vertices = verts.shape[0]
+        sd.vertices = verts.ctypes.data_as(C.POINTER(C.c_float))
+        sd.normals = norms.ctypes.data_as(C.POINTER(C.c_float))
+        sd.texcoords = uvs.ctypes.data_as(C.POINTER(C.c_float))
+        sd.num_faces = faces.shape[0]
+        sd.indices = faces.ctypes.data_as(C.POINTER(C.c_uint32))
+        sd.face_materials = mats.ctypes.data_as(C.POINTER(C.c_uint16))
+        arr = (api.InstanceDesc * max(1, len(inst)))()
+        for k, d in enumerate(inst):
+            arr[k] = api.InstanceDesc(d["vertex_offset"], d["vertex_count"], d["face_offset"], d["face_count"],
+                                      (C.c_float * 16)(*d["transform"]), d["visibility"], d["emissive"])
+        self._keep.append(arr)
+        sd.num_instances = len(inst)
+        sd.instances = arr
+
+        def light_array(ctype, items):
+            a = (ctype * max(1, len(items)))(*items)
+            self._keep.append(a)
+            return len(items), a
+
+        sd.num_area_lights, sd.area_lights = light_array(api.AreaLight, self.area_lights)
+        sd.num_point_lights, sd.point_lights = light_array(api.PointLight, self.point_lights)
+        sd.num_sun_lights, sd.sun_lights = light_array(api.SunLight, self.sun_lights)
+        sd.num_spot_lights, sd.spot_lights = light_array(api.SpotLight, self.spot_lights)
+        sd.world_shader = self.world_shader
+        sd.envmap_resolution = self.envmap_resolution
+        sd.camera = self.camera_c()
+        return sd
+
+    @property
+    def num_triangles(self):
+        return sum(self.shapes[e[0]].faces.shape[0] for e in self.entities)
+
+
+# ---------------------------------------------------------------------------
+# Geometry builders
+
+
+def quad(p0, p1, p2, p3, material):
+    """Two triangles (p0 p1 p2), (p0 p2 p3) with a flat normal."""
+    v = np.array([p0, p1, p2, p3], np.float64)
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    n /= np.linalg.norm(n)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    return v, np.repeat(n[None], 4, 0), np.array([[0, 1, 2], [0, 2, 3]]), [material, material], uv
+
+
+def merge(parts):
+    vs, ns, fs, ms, uvs = [], [], [], [], []
+    off = 0
+    for (v, n, f, m, uv) in parts:
+        vs.append(v)
+        ns.append(n)
+        fs.append(f + off)
+        ms.extend(m)
+        uvs.append(uv)
+        off += v.shape[0]
+    return Shape(np.concatenate(vs), np.concatenate(ns), np.concatenate(fs), ms, np.concatenate(uvs))
+
+
+def unit_cube(material):
+    """Cube [-0.5, 0.5]^3 with outward normals, 12 triangles."""
+    h = 0.5
+    P = lambda x, y, z: (x * h, y * h, z * h)
+    faces = [
+        quad(P(-1, -1, -1), P(-1, 1, -1), P(1, 1, -1), P(1, -1, -1), material),   # bottom (-z)
+        quad(P(-1, -1, 1), P(1, -1, 1), P(1, 1, 1), P(-1, 1, 1), material),       # top (+z)
+        quad(P(-1, -1, -1), P(1, -1, -1), P(1, -1, 1), P(-1, -1, 1), material),   # front (-y)
+        quad(P(1, 1, -1), P(-1, 1, -1), P(-1, 1, 1), P(1, 1, 1), material),       # back (+y)
+        quad(P(-1, 1, -1), P(-1, -1, -1), P(-1, -1, 1), P(-1, 1, 1), material),   # left (-x)
+        quad(P(1, -1, -1), P(1, 1, -1), P(1, 1, 1), P(1, -1, 1), material),       # right (+x)
+    ]
+    return merge(faces)
+
+
+def cornell_box(width=512, height=512):
+    """Config 1: unit Cornell box, 34 triangles + a 0.5x0.5 emitter (2 triangles), z up."""
+    d = SceneDescription("cornell")
+    WHITE, RED, GREEN, TALL, LIGHT = 0, 1, 2, 3, 4
+    d.surface_materials = [
+        diffuse_material("white", (0.73, 0.73, 0.73)),
+        diffuse_material("red", (0.65, 0.05, 0.05)),
+        diffuse_material("green", (0.12, 0.45, 0.15)),
+        principled_material("tall box", Base_Color=[0.73, 0.73, 0.73, 1.0], Metallic=0.0, Roughness=0.4,
+                            Specular=0.5),
+        emission_material("light", (1.0, 1.0, 1.0), 15.0),
+    ]
+    d.light_materials = [background_world("world", (0.0, 0.0, 0.0), 1.0)]
+    d.world_shader = 0
+    walls = merge([
+        quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), WHITE),   # floor, normal +z
+        quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), WHITE),   # ceiling, normal -z
+        quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), WHITE),   # back wall, normal -y
+        quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), RED),     # left wall, normal +x
+        quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), GREEN),   # right wall, normal -x
+    ])
+    s_walls = d.add_shape(walls)
+    s_short = d.add_shape(unit_cube(WHITE))
+    s_tall = d.add_shape(unit_cube(TALL))
+    light = merge([quad((-0.5, -0.5, 0), (-0.5, 0.5, 0), (0.5, 0.5, 0), (0.5, -0.5, 0), LIGHT)])  # normal -z
+    s_light = d.add_shape(light)
+    d.add_entity(s_walls)
+    d.add_entity(s_short, translate((0.66, 0.32, 0.15)) @ rotate_z(-17.0) @ scale((0.3, 0.3, 0.3)))
+    d.add_entity(s_tall, translate((0.34, 0.64, 0.3)) @ rotate_z(17.0) @ scale((0.3, 0.3, 0.6)))
+    d.add_entity(s_light, translate((0.5, 0.5, 0.999)) @ scale((0.5, 0.5, 1.0)), emissive=True)
+    d.camera = dict(transform=look_at((0.5, -1.35, 0.5), (0.5, 0.5, 0.5)), near=1e-3, far=1e4, focal=2.4)
+    d.width, d.height = width, height
+    return d
+
+
+def icosphere(level):
+    """Unit icosphere by midpoint subdivision: 10*4^level + 2 vertices, 20*4^level faces."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+                  [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                  [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        a, b, c = inv[:nf] + base, inv[nf:2 * nf] + base, inv[2 * nf:] + base
+        f = np.concatenate([np.stack([f[:, 0], a, c], 1), np.stack([f[:, 1], b, a], 1),
+                            np.stack([f[:, 2], c, b], 1), np.stack([a, b, c], 1)])
+    return v, f
+
+
+def value_noise(p, seed, octaves=3):
+    """Seeded 3-D value noise in [-1, 1]: hashed lattice values, smoothstep-trilinear, fBm octaves."""
+    out = np.zeros(p.shape[0])
+    amp, freq, norm = 1.0, 4.0, 0.0
+    for o in range(octaves):
+        q = p * freq
+        i = np.floor(q).astype(np.int64)
+        fr = q - i
+        w = fr * fr * (3.0 - 2.0 * fr)
+
+        def lattice(dx, dy, dz):
+            h = ((i[:, 0] + dx) * 73856093) ^ ((i[:, 1] + dy) * 19349663) ^ ((i[:, 2] + dz) * 83492791)
+            h = (h ^ (seed * 2654435761 + o * 97531)) & 0xFFFFFFFF
+            h = (h * 2246822519) & 0xFFFFFFFF
+            h ^= h >> 13
+            h = (h * 3266489917) & 0xFFFFFFFF
+            h ^= h >> 16
+            return h / 4294967295.0 * 2.0 - 1.0
+
+        acc = 0.0
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    wx = w[:, 0] if dx else 1 - w[:, 0]
+                    wy = w[:, 1] if dy else 1 - w[:, 1]
+                    wz = w[:, 2] if dz else 1 - w[:, 2]
+                    acc = acc + wx * wy * wz * lattice(dx, dy, dz)
+        out += amp * acc
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return out / norm
+
+
+def vertex_normals(v, f):
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    n = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n
+
+
+def sphere_field(width=1920, height=1080, grid=8, level=6, radius=0.4, amplitude=0.05):
+    """Config 2: grid x grid displaced icospheres (seed 1000+i), jittered grid (seed 7),
+    Principled per instance (seed 42), ground plane, analytic sky."""
+    d = SceneDescription("spheres")
+    rng_mat = np.random.default_rng(42)
+    mats = []
+    n = grid * grid
+    for i in range(n):
+        base = rng_mat.uniform(0, 1, 3)
+        metallic = 1.0 if rng_mat.uniform() < 0.3 else 0.0
+        rough = rng_mat.uniform(0.05, 1.0)
+        coat = rng_mat.uniform(0, 1)
+        mats.append(principled_material("sphere %02d" % i, Base_Color=[float(x) for x in base] + [1.0],
+                                        Metallic=metallic, Roughness=float(rough), Specular=0.5,
+                                        Clearcoat=float(coat)))
+    mats.append(diffuse_material("ground", (0.5, 0.5, 0.5)))
+    d.surface_materials = mats
+    d.light_materials = [sky_world()]
+    d.world_shader = 0
+    v0, f0 = icosphere(level)
+    rng_grid = np.random.default_rng(7)
+    jitter = rng_grid.uniform(-0.25, 0.25, (n, 2))
+    for i in range(n):
+        disp = radius * (1.0 + amplitude * value_noise(v0, 1000 + i) / radius)
+        v = v0 * disp[:, None]
+        s = Shape(v, vertex_normals(v, f0), f0, i)
+        si = d.add_shape(s)
+        gx, gy = i % grid, i // grid
+        cx = (gx - (grid - 1) / 2.0) + jitter[i, 0]
+        cy = (gy - (grid - 1) / 2.0) + jitter[i, 1]
+        d.add_entity(si, translate((cx, cy, radius)))
+    g = 50.0
+    ground = merge([quad((-g, -g, 0), (g, -g, 0), (g, g, 0), (-g, g, 0), n)])
+    d.add_entity(d.add_shape(ground))
+    d.camera = dict(transform=look_at((0.0, -8.5, 5.5), (0.0, 0.3, 0.0)), near=1e-3, far=1e4, focal=2.0)
+    d.width, d.height = width, height
+    return d
+
+
+def build_device_scene(device, desc, threads=0, envmap=True):
+    """SceneLoader.loadScene: materials (JIT), shapes/entities (upload + BVH), lights (envmap)."""
+    scene = api.Scene(device, desc)
+    scene.compile_materials(desc.material_source())
+    scene.build_accel(threads)
+    if envmap:
+        scene.build_envmap()
+    return scene
