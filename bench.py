@@ -1,0 +1,148 @@
+"""Benchmark: Mrays/s of the wavefront path tracer on one MI355X.
+
+Workload (configuration 2): 64 level-6 icospheres displaced by 3-octave value
+noise plus a ground plane (5,242,882 triangles), Principled materials, an
+analytic sky importance-sampled through a 2048^2 envmap, 1920x1080, path
+depth 12.  One step = one sample per pixel (2,073,600 paths) traced through
+all bounces.  Mrays/s counts closest-hit plus shadow rays traced, the
+reference's own accounting (host/Renderer.swift:190-196,249-253).
+
+Prints one JSON line.  `--gpus` above 1 is rejected: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES = 80, 48
+RAY_READ, HIT_WRITE = 28, 20  # origin + direction + meta word; t, prim, inst, 2 barycentrics
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--spp-per-launch", type=int, default=1, help="path-queue capacity in frames")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-pixels", type=int, default=0, help="oracle sample size (0 = full frames)")
+    ap.add_argument("--cpu-frames", type=int, default=16, help="oracle sample: frames (spp) of the full image")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU (the path tracer is single-GPU in this build); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import raymond_amd.api as api
+    import raymond_amd.scenes as scenes
+
+    W, H, D = args.width, args.height, args.depth
+    t0 = time.time()
+    desc = scenes.sphere_field(W, H)
+    dev = api.Device(0)
+    scene = scenes.build_device_scene(dev, desc)
+    sst = scene.stats()
+    setup_s = time.time() - t0
+    print("setup %.1fs: %d triangles, BVH8 %d nodes depth %d (%.1fs), JIT %.1fs, envmap %.2fs" %
+          (setup_s, sst.triangles, sst.bvh_nodes, sst.bvh_depth, sst.bvh_build_seconds, sst.jit_seconds,
+           sst.envmap_seconds), file=sys.stderr, flush=True)
+
+    spl = args.spp_per_launch
+    r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
+    r.set_uniforms(timing=1, rr_depth=0)
+    if args.warmup:
+        r.execute(args.warmup)
+    # the timed steps run as one regenerating stream of `steps` samples per pixel
+    t_start = time.perf_counter()
+    st = r.execute(args.steps)
+    wall = time.perf_counter() - t_start
+    totals = dict(rays=st.total_rays, closest=st.closest_total, shadow=st.shadow_total,
+                  ms_closest=st.ms_trace_closest, ms_shade=st.ms_shade, ms_any=st.ms_trace_any,
+                  nclosest=st.trace_closest_launches, nany=st.trace_any_launches, iters=st.iterations)
+    closest_total = st.closest_total
+    mrays = totals["rays"] / wall / 1e6
+    samples_per_s = W * H * args.steps / wall
+
+    # traversal statistics pass (not timed): nodes and triangles per ray
+    r.set_uniforms(traversal_stats=1, timing=0)
+    sst2 = r.execute(1)
+    closest_rays = sst2.closest_total
+    nodes_per_ray = sst2.nodes_closest / max(closest_rays, 1)
+    tris_per_ray = sst2.tris_closest / max(closest_rays, 1)
+    bytes_per_ray = RAY_READ + HIT_WRITE + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
+    closest_bytes = closest_total * bytes_per_ray
+    achieved = closest_bytes / (totals["ms_closest"] * 1e-3) / 1e9 if totals["ms_closest"] > 0 else 0.0
+    print("timed %d steps in %.3fs: %.1f Mrays/s, %.1f Msamples/s; closest %.2f ms/step, shade %.2f, any %.2f; "
+          "nodes/ray %.1f tris/ray %.1f" % (args.steps, wall, mrays, samples_per_s / 1e6,
+                                            totals["ms_closest"] / args.steps, totals["ms_shade"] / args.steps,
+                                            totals["ms_any"] / args.steps, nodes_per_ray, tris_per_ray),
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        orc = binding.OracleScene(desc)
+        pdfs, mip = scene.read_envmap()
+        orc.set_envmap(pdfs, mip)
+        if args.cpu_pixels:
+            rng = np.random.default_rng(11)
+            pix = rng.choice(W * H, args.cpu_pixels, replace=False).astype(np.uint32)
+            frames = 1
+            sample = "%d random pixels of the %dx%d frame, 1 spp, depth %d" % (args.cpu_pixels, W, H, D)
+        else:
+            pix = None
+            frames = args.cpu_frames
+            sample = "%d full %dx%d frames (%d bench steps), 1 spp each, depth %d" % (frames, W, H, frames, D)
+        _, res = orc.render(W, H, D, frames=frames, frame_begin=10_000, pixels=pix, threads=threads)
+        cpu = {"value": res.total_rays / res.seconds / 1e6, "unit": "Mrays/s", "cores": int(res.threads_used),
+               "kind": "port", "sample": sample, "seconds": round(res.seconds, 2)}
+        print("cpu oracle: %.3f Mrays/s on %d threads (%.1fs)" % (cpu["value"], cpu["cores"], res.seconds),
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "Mrays/s + RMSE vs high-spp reference at equal time, cache vs plain",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "sphere field (config 2): 64 noise-displaced level-6 icospheres + ground, 5,242,882 "
+                        "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D),
+            "width": W, "height": H, "max_depth": D, "queue_frames": spl, "triangles": int(sst.triangles),
+            "iterations": int(totals["iters"]),
+        },
+        "samples_per_s": round(samples_per_s, 1),
+        "stage_ms_per_step": {"trace_closest": round(totals["ms_closest"] / args.steps, 3),
+                              "shade": round(totals["ms_shade"] / args.steps, 3),
+                              "trace_any_shadow": round(totals["ms_any"] / args.steps, 3)},
+        "roofline": {"kernel": "trace_closest", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_ray": round(bytes_per_ray, 1),
+                     "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2)},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

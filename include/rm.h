@@ -210,9 +210,14 @@ typedef struct rm_uniforms {
 } rm_uniforms;
 
 typedef struct rm_render_stats {
-    uint64_t rays[33];          /* closest-hit rays traced per depth (rayCounts) */
-    uint64_t shadow_rays[32];   /* shadow rays traced per depth (shadowRayCounts) */
-    uint64_t total_rays;        /* sum of both */
+    /* Per bounce-loop iteration.  With samples <= samples_per_launch no path
+     * regeneration happens and iteration == depth, i.e. the reference's
+     * rayCounts / shadowRayCounts (Renderer.swift:190-196). */
+    uint64_t rays[33];          /* closest-hit rays traced */
+    uint64_t shadow_rays[32];   /* shadow rays traced */
+    uint64_t total_rays;        /* closest_total + shadow_total */
+    uint64_t closest_total;     /* all closest-hit rays traced */
+    uint64_t shadow_total;      /* all shadow rays traced */
     uint64_t samples;           /* paths started */
     double ms_total;            /* wall time of the rm_render call */
     double ms_trace_closest;    /* stage GPU times when timing != 0 */
@@ -220,6 +225,7 @@ typedef struct rm_render_stats {
     double ms_trace_any;
     uint32_t trace_closest_launches;
     uint32_t trace_any_launches;
+    uint64_t iterations;        /* bounce-loop iterations */
     uint64_t nodes_closest, tris_closest, nodes_any, tris_any; /* traversal_stats */
 } rm_render_stats;
 
@@ -231,8 +237,10 @@ void rm_renderer_destroy(rm_renderer* r);
 int rm_renderer_set_uniforms(rm_renderer* r, const rm_uniforms* u);
 /* Clears the accumulation and the frame counter (Renderer.reset). */
 int rm_renderer_reset(rm_renderer* r);
-/* Renders `samples` more samples per pixel (Renderer.execute, one call per
- * launch of samples_per_launch).  Blocks until the GPU is done. */
+/* Renders `samples` more samples per pixel (Renderer.execute) as one
+ * regenerating stream: the path queue holds width*height*samples_per_launch
+ * paths and is refilled with new camera paths as paths terminate.  Blocks
+ * until the GPU is done. */
 int rm_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
 /* Accumulated RGBA sums (alpha counts contributions) -- width*height*4 floats. */
 int rm_read_image(rm_renderer* r, float* rgba, size_t count);

@@ -90,10 +90,10 @@ class Uniforms(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("rays", C.c_uint64 * 33), ("shadow_rays", C.c_uint64 * 32), ("total_rays", C.c_uint64),
-                ("samples", C.c_uint64), ("ms_total", C.c_double), ("ms_trace_closest", C.c_double),
+                ("closest_total", C.c_uint64), ("shadow_total", C.c_uint64), ("samples", C.c_uint64), ("ms_total", C.c_double), ("ms_trace_closest", C.c_double),
                 ("ms_shade", C.c_double), ("ms_trace_any", C.c_double),
                 ("trace_closest_launches", C.c_uint32), ("trace_any_launches", C.c_uint32),
-                ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_any", C.c_uint64),
+                ("iterations", C.c_uint64), ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_any", C.c_uint64),
                 ("tris_any", C.c_uint64)]
 
 

@@ -412,20 +412,22 @@ struct UberShader {
         wo = wo * to_local;
         if (wo.z * wo_geo < 0.0f) return BsdfSample::invalid();
 
-        int lobe;
+        // lobe probabilities as values, not addresses: keeps the mixture in registers
+        const float lp0 = lobe_prob[0], lp1 = lobe_prob[1], lp2 = lobe_prob[2], lp3 = lobe_prob[3];
         BsdfSample s;
+        float p_lobe;
         const vec2 r2(rnd.y, rnd.z);
-        if (rnd.x < lobe_prob[0]) {
-            lobe = 0; s = diffuse.sample(r2, wo);
-        } else if (rnd.x < (lobe_prob[0] + lobe_prob[1])) {
-            lobe = 1; s = specular.sample(r2, wo);
-        } else if (rnd.x < ((lobe_prob[0] + lobe_prob[1]) + lobe_prob[2])) {
-            lobe = 2; s = transmission.sample(r2, wo);
+        if (rnd.x < lp0) {
+            p_lobe = lp0; s = diffuse.sample(r2, wo);
+        } else if (rnd.x < (lp0 + lp1)) {
+            p_lobe = lp1; s = specular.sample(r2, wo);
+        } else if (rnd.x < ((lp0 + lp1) + lp2)) {
+            p_lobe = lp2; s = transmission.sample(r2, wo);
         } else {
-            lobe = 3; s = clearcoat.sample(r2, wo);
+            p_lobe = lp3; s = clearcoat.sample(r2, wo);
         }
-        if (!isfinite_(lobe_prob[lobe])) return BsdfSample::invalid();
-        if (lobe_prob[lobe] < 1.0f) {
+        if (!isfinite_(p_lobe)) return BsdfSample::invalid();
+        if (p_lobe < 1.0f) {
             // MIS needs the whole mixture, not just the sampled lobe (UberShader.hpp:127-131)
             s.weight = evaluate_local(wo, s.wi, s.pdf);
             s.weight = s.weight / s.pdf;

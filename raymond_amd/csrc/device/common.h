@@ -11,7 +11,7 @@
 #pragma once
 
 #if defined(__HIPCC_RTC__)
-#define RM_HD __host__ __device__ inline
+#define RM_HD __host__ __device__ inline __attribute__((always_inline))
 #define RM_GPU 1
 typedef unsigned char uint8_t;
 typedef unsigned short uint16_t;
@@ -22,7 +22,7 @@ typedef long long int64_t;
 #elif defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#define RM_HD __host__ __device__ inline
+#define RM_HD __host__ __device__ inline __attribute__((always_inline))
 #define RM_GPU 1
 #else
 #include <stdint.h>

@@ -46,13 +46,24 @@ struct ShadowQueue {
     uint32_t* pixel;
 };
 
-// Per-launch device counters (zeroed by a kernel at launch start).
+// Per-iteration counters of the regenerating bounce loop.  Iteration i:
+// regenerate(i) tops the queue up with new camera paths, then trace, shade
+// (appends survivors for iteration i+1) and shadow trace run on it.
+struct IterCounters {
+    uint32_t queue;          // paths traced in iteration i (survivors + new)
+    uint32_t appended;       // survivors appended by shade(i-1)
+    uint32_t generated;      // camera paths generated before iteration i
+    uint32_t shadows;        // shadow rays emitted by shade(i)
+    uint32_t fetch_closest;  // persistent-scheduler counters
+    uint32_t fetch_any;
+    uint32_t pad0, pad1;
+};
+
+constexpr int kMaxIters = 2048;
+
+// Device counters of one render stream (zeroed by a kernel at stream start).
 struct Counters {
-    uint32_t rays[kMaxDepth + 1];      // paths entering each depth
-    uint32_t shadows[kMaxDepth];       // shadow rays emitted at each depth
-    uint32_t fetch_rays[kMaxDepth + 1];
-    uint32_t fetch_shadows[kMaxDepth];
-    uint32_t shade_done[kMaxDepth];
+    IterCounters it[kMaxIters];
     // traversal statistics (only written by the statistics variants)
     unsigned long long nodes_closest, tris_closest, nodes_any, tris_any;
 };
@@ -60,7 +71,7 @@ struct Counters {
 // Render-time settings (bridge/Uniforms.hpp:29-47 plus this build's knobs).
 struct FrameUniforms {
     uint32_t width, height;
-    uint32_t samples;          // samples traced together in this launch
+    uint32_t samples;          // samples per pixel in this stream
     uint32_t random_seed;      // seed of sample 0 (frame index + 1)
     uint32_t sampling_mode;    // 0 bsdf, 1 nee, 2 mis
     uint32_t rr_depth;         // roulette starts at this depth (0 = reference behaviour)

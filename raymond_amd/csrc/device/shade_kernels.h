@@ -68,19 +68,41 @@ __device__ inline void image_add(float* image, uint32_t pixel, vec3 c) {
     atomicAdd(p + 3, 1.0f);
 }
 
-// Wave-level compaction: one atomic per wave, slots in lane order.  Must be
-// called with every lane of the wave active.
-__device__ inline uint32_t wave_append(uint32_t* counter, bool want) {
-    const unsigned long long mask = __ballot(want);
+constexpr uint32_t kShadeBlock = 256;
+constexpr uint32_t kShadeWaves = kShadeBlock / 64;
+
+// Workgroup-level compaction of two append streams: ballots give each lane's
+// rank inside its wave, LDS prefix sums give the wave offsets, and one atomic
+// per stream and workgroup reserves the slots (slots in lane order).  Must be
+// reached by every thread of the workgroup.
+__device__ inline void block_append2(uint32_t* c0, bool w0, uint32_t* c1, bool w1, uint32_t& s0, uint32_t& s1) {
+    __shared__ uint32_t wave_cnt[2][kShadeWaves];
+    __shared__ uint32_t base[2];
     const uint32_t lane = __lane_id();
-    const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (mask != 0ull) {
-        const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
-        if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-        base = __shfl(base, (int)leader);
+    const uint32_t wave = threadIdx.x / 64u;
+    const unsigned long long m0 = __ballot(w0), m1 = __ballot(w1);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    if (lane == 0) {
+        wave_cnt[0][wave] = (uint32_t)__popcll(m0);
+        wave_cnt[1][wave] = (uint32_t)__popcll(m1);
     }
-    return base + before;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t0 = 0, t1 = 0;
+        for (uint32_t w = 0; w < kShadeWaves; w++) {
+            const uint32_t a = wave_cnt[0][w], b = wave_cnt[1][w];
+            wave_cnt[0][w] = t0;
+            wave_cnt[1][w] = t1;
+            t0 += a;
+            t1 += b;
+        }
+        base[0] = t0 ? atomicAdd(c0, t0) : 0u;
+        base[1] = t1 ? atomicAdd(c1, t1) : 0u;
+    }
+    __syncthreads();
+    s0 = base[0] + wave_cnt[0][wave] + (uint32_t)__popcll(m0 & lt);
+    s1 = base[1] + wave_cnt[1][wave] + (uint32_t)__popcll(m1 & lt);
+    __syncthreads();
 }
 
 // One path vertex (device/kernels/shading/intersection.metal:43-208).
@@ -149,6 +171,9 @@ __device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out)
         sh_normal = ensure_valid_reflection(geo, shading.wo, shading.material.normal);
     }
 
+    // the last depth's shadow and continuation rays are never traced
+    // (Renderer.swift:623-625), so they are not emitted
+    const bool last = depth + 1u >= a.u.max_depth;
     if (mode != 0u) {
         const LightSample nee = lights_sample(ctx, shading, prng);
         float bsdf_pdf;
@@ -157,7 +182,7 @@ __device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out)
         if (nee.casts_shadows) {
             const float mis = (mode == 1u || !nee.can_be_hit) ? 1.0f : mis_weight(nee.pdf, bsdf_pdf);
             const vec3 nee_weight = mis * contribution;
-            if (all_finite(nee_weight) && any_nonzero(nee_weight)) {
+            if (all_finite(nee_weight) && any_nonzero(nee_weight) && !last) {
                 out.shadow = true;
                 out.s_origin = shading.position;
                 out.s_dir = nee.direction;
@@ -183,7 +208,7 @@ __device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out)
         survival = min_(mean_w, 1.0f);
         survive = prng.sample() < survival;
     }
-    if (survive) {
+    if (survive && !last) {
         out.next = true;
         out.n_origin = shading.position;
         out.n_dir = s.wi;
@@ -194,14 +219,16 @@ __device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out)
     }
 }
 
-extern "C" __global__ __launch_bounds__(256) void rm_shade_kernel(ShadeArgs a) {
-    const uint32_t count = a.counters->rays[a.depth];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) {
+    const uint32_t count = a.counters->it[a.iter].queue;
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         ShadeOut out;
         if (i < count) shade_path(a, i, out);
-        const uint32_t sslot = wave_append(&a.counters->shadows[a.depth], out.shadow);
+        uint32_t sslot, nslot;
+        block_append2(&a.counters->it[a.iter].shadows, out.shadow, &a.counters->it[a.iter + 1].appended, out.next,
+                      sslot, nslot);
         if (out.shadow) {
             a.shadows.ox[sslot] = out.s_origin.x;
             a.shadows.oy[sslot] = out.s_origin.y;
@@ -215,7 +242,6 @@ extern "C" __global__ __launch_bounds__(256) void rm_shade_kernel(ShadeArgs a) {
             a.shadows.wz[sslot] = out.s_weight.z;
             a.shadows.pixel[sslot] = out.pixel;
         }
-        const uint32_t nslot = wave_append(&a.counters->rays[a.depth + 1], out.next);
         if (out.next) {
             a.next.ox[nslot] = out.n_origin.x;
             a.next.oy[nslot] = out.n_origin.y;

@@ -16,7 +16,7 @@ struct ShadeArgs {
     Counters* counters;
     float* image;          // RGBA accumulation
     FrameUniforms u;
-    uint32_t depth;
+    uint32_t iter;             // iteration of the regenerating bounce loop
 };
 
 struct EnvmapArgs {

@@ -25,7 +25,7 @@ static void carve_path_queue(DevBuf& buf, size_t cap, PathQueue& q) {
 
 void renderer_create(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, uint32_t spl, rm_renderer* r) {
     RM_REQUIRE(w > 0 && h > 0, "image size must be positive");
-    RM_REQUIRE(max_depth >= 1 && max_depth <= (uint32_t)kMaxDepth, "max_depth out of range");
+    RM_REQUIRE(max_depth >= 1 && max_depth <= 255, "max_depth out of range");
     RM_REQUIRE(spl >= 1, "samples_per_launch must be >= 1");
     RM_REQUIRE((uint64_t)w * h * spl < (1ull << 31), "wavefront too large");
     r->scene = s;
@@ -110,7 +110,13 @@ struct StageTimer {
 
 }  // namespace
 
-static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
+// One regenerating stream of `samples` samples per pixel.  Iteration i runs
+// regenerate(i) -> closest(i) -> shade(i) -> shadow(i); queue lengths live in
+// device counters, and the host checks for completion once every
+// kIterChunk iterations.
+static constexpr uint32_t kIterChunk = 8;
+
+static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     rm_scene* s = r->scene;
     rm_device* dev = s->dev;
     hipStream_t st = dev->stream;
@@ -118,27 +124,38 @@ static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
     const bool tstats = r->u.traversal_stats != 0;
     Counters* cnt = r->counters_buf.as<Counters>();
     float* image = r->image_buf.as<float>();
-    const int tblocks = dev->props.multiProcessorCount * dev->trace_blocks_per_cu;
-    const int ablocks = dev->props.multiProcessorCount * dev->any_blocks_per_cu;
+    const int cus = dev->props.multiProcessorCount;
+    const int tblocks = cus * dev->trace_blocks_per_cu;
+    const int ablocks = cus * dev->any_blocks_per_cu;
     int shade_bpc = 0;
     RM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&shade_bpc, s->jit.shade, 256, 0));
     if (shade_bpc < 1) shade_bpc = 1;
-    const int sblocks = dev->props.multiProcessorCount * shade_bpc;
+    const int sblocks = cus * shade_bpc;
+    const uint32_t total = W * H * samples;
 
     StageTimer tm{r->u.timing != 0, st, {}, {}, {}, r->events};
-
     if (!r->u.accumulate) RM_HIP(hipMemsetAsync(image, 0, r->image_buf.bytes, st));
-    RM_HIP(launch_clear_counters(cnt, W * H * n, st));
+    RM_HIP(launch_clear_counters(cnt, st));
 
     FrameUniforms fu{};
     fu.width = W;
     fu.height = H;
-    fu.samples = n;
+    fu.samples = samples;
     fu.random_seed = r->frame_index + 1;
     fu.sampling_mode = r->u.sampling_mode;
     fu.rr_depth = r->u.rr_depth;
     fu.max_depth = r->max_depth;
     fu.image_offset = 0;
+
+    RegenArgs ga{};
+    ga.counters = cnt;
+    ga.capacity = (uint32_t)r->capacity;
+    ga.total = total;
+    ga.camera = s->view.camera;
+    ga.width = W;
+    ga.height = H;
+    ga.random_seed = fu.random_seed;
+    ga.image_offset = 0;
 
     ClosestArgs ca{};
     ca.nodes = s->d_nodes.as<Bvh8Node>();
@@ -147,61 +164,55 @@ static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
     ca.spill = s->d_spill.as<uint2>();
     ca.counters = cnt;
     ca.camera = s->view.camera;
-    ca.width = W;
-    ca.height = H;
-    ca.random_seed = fu.random_seed;
-    ca.image_offset = 0;
 
-    // depth 0: camera rays generated and traced in one kernel (generate.metal:72-192)
-    ca.mode = kTraceCamera;
-    ca.count = W * H * n;
-    ca.count_ptr = nullptr;
-    ca.fetch = &cnt->fetch_rays[0];
-    ca.paths = r->q[0];
-    hipEvent_t e0 = tm.begin();
-    RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
-    tm.end(e0, tm.closest);
+    AnyArgs aa{};
+    aa.nodes = ca.nodes;
+    aa.tris = ca.tris;
+    aa.shadows = r->shadows;
+    aa.image = image;
+    aa.spill = ca.spill;
+    aa.counters = cnt;
 
-    for (uint32_t depth = 0; depth < r->max_depth; depth++) {
-        PathQueue& cur = r->q[depth & 1];
-        PathQueue& next = r->q[(depth + 1) & 1];
-        if (depth > 0) {
-            ca.mode = kTraceQueue;
-            ca.count_ptr = &cnt->rays[depth];
-            ca.fetch = &cnt->fetch_rays[depth];
+    uint32_t iter = 0;
+    std::vector<IterCounters> host_it;
+    for (;;) {
+        RM_REQUIRE(iter + kIterChunk + 1 < (uint32_t)kMaxIters, "render stream exceeded the iteration limit");
+        for (uint32_t c = 0; c < kIterChunk; c++, iter++) {
+            PathQueue& cur = r->q[iter & 1];
+            PathQueue& next = r->q[(iter + 1) & 1];
+            ga.iter = iter;
+            ga.paths = cur;
+            RM_HIP(launch_regenerate(ga, cus * 4, st));
+            ca.count_ptr = &cnt->it[iter].queue;
+            ca.fetch = &cnt->it[iter].fetch_closest;
             ca.paths = cur;
             hipEvent_t e = tm.begin();
             RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
             tm.end(e, tm.closest);
+            ShadeArgs sa;
+            sa.ctx = s->view;
+            sa.cur = cur;
+            sa.hits = r->hits;
+            sa.next = next;
+            sa.shadows = r->shadows;
+            sa.counters = cnt;
+            sa.image = image;
+            sa.u = fu;
+            sa.iter = iter;
+            void* args[] = {&sa};
+            hipEvent_t es = tm.begin();
+            RM_HIP(hipModuleLaunchKernel(s->jit.shade, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
+            tm.end(es, tm.shade);
+            aa.count_ptr = &cnt->it[iter].shadows;
+            aa.fetch = &cnt->it[iter].fetch_any;
+            hipEvent_t ea = tm.begin();
+            RM_HIP(launch_trace_any(aa, ablocks, tstats, st));
+            tm.end(ea, tm.any);
         }
-        ShadeArgs sa;
-        sa.ctx = s->view;
-        sa.cur = cur;
-        sa.hits = r->hits;
-        sa.next = next;
-        sa.shadows = r->shadows;
-        sa.counters = cnt;
-        sa.image = image;
-        sa.u = fu;
-        sa.depth = depth;
-        void* args[] = {&sa};
-        hipEvent_t es = tm.begin();
-        RM_HIP(hipModuleLaunchKernel(s->jit.shade, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
-        tm.end(es, tm.shade);
-        // shadow rays of the last depth are never traced (Renderer.swift:623-625)
-        if (depth + 1 == r->max_depth) break;
-        AnyArgs aa{};
-        aa.nodes = ca.nodes;
-        aa.tris = ca.tris;
-        aa.count_ptr = &cnt->shadows[depth];
-        aa.fetch = &cnt->fetch_shadows[depth];
-        aa.shadows = r->shadows;
-        aa.image = image;
-        aa.spill = ca.spill;
-        aa.counters = cnt;
-        hipEvent_t ea = tm.begin();
-        RM_HIP(launch_trace_any(aa, ablocks, tstats, st));
-        tm.end(ea, tm.any);
+        IterCounters probe{};
+        RM_HIP(hipMemcpyAsync(&probe, &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));
+        RM_HIP(hipStreamSynchronize(st));
+        if (probe.generated >= total && probe.appended == 0) break;
     }
     if (r->u.accumulate_f64) {
         if (!r->image64_buf.ptr) {
@@ -210,20 +221,22 @@ static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
         }
         RM_HIP(launch_accum_f64(image, r->image64_buf.as<double>(), W * H * 4, st));
     }
-    Counters host{};
-    RM_HIP(hipMemcpyAsync(&host, cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    host_it.resize(iter);
+    RM_HIP(hipMemcpyAsync(host_it.data(), cnt->it, iter * sizeof(IterCounters), hipMemcpyDeviceToHost, st));
+    unsigned long long tstat[4] = {0, 0, 0, 0};
+    RM_HIP(hipMemcpyAsync(tstat, &cnt->nodes_closest, sizeof(tstat), hipMemcpyDeviceToHost, st));
     RM_HIP(hipStreamSynchronize(st));
-    r->frame_index += n;
+    r->frame_index += samples;
     if (stats) {
-        for (uint32_t d = 0; d < r->max_depth; d++) {
-            stats->rays[d] += host.rays[d];
-            stats->total_rays += host.rays[d];
-            if (d + 1 < r->max_depth) {
-                stats->shadow_rays[d] += host.shadows[d];
-                stats->total_rays += host.shadows[d];
-            }
+        for (uint32_t i = 0; i < iter; i++) {
+            if (i < 33) stats->rays[i] += host_it[i].queue;
+            if (i < 32) stats->shadow_rays[i] += host_it[i].shadows;
+            stats->total_rays += (uint64_t)host_it[i].queue + host_it[i].shadows;
+            stats->closest_total += host_it[i].queue;
+            stats->shadow_total += host_it[i].shadows;
         }
-        stats->samples += (uint64_t)W * H * n;
+        stats->samples += (uint64_t)total;
+        stats->iterations += iter;
         stats->trace_closest_launches += (uint32_t)tm.closest.size();
         stats->trace_any_launches += (uint32_t)tm.any.size();
         if (tm.on) {
@@ -231,10 +244,10 @@ static void render_launch(rm_renderer* r, uint32_t n, rm_render_stats* stats) {
             stats->ms_shade += StageTimer::sum(tm.shade);
             stats->ms_trace_any += StageTimer::sum(tm.any);
         }
-        stats->nodes_closest += host.nodes_closest;
-        stats->tris_closest += host.tris_closest;
-        stats->nodes_any += host.nodes_any;
-        stats->tris_any += host.tris_any;
+        stats->nodes_closest += tstat[0];
+        stats->tris_closest += tstat[1];
+        stats->nodes_any += tstat[2];
+        stats->tris_any += tstat[3];
     }
 }
 
@@ -245,10 +258,12 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     RM_REQUIRE(s->envmap_built, "envmap not built");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const auto t0 = std::chrono::steady_clock::now();
+    // bound a stream so its iteration count stays within kMaxIters
+    const uint32_t max_stream = 512;
     uint32_t left = samples;
     while (left > 0) {
-        const uint32_t n = left < r->spl ? left : r->spl;
-        render_launch(r, n, stats);
+        const uint32_t n = left < max_stream ? left : max_stream;
+        render_stream(r, n, stats);
         left -= n;
     }
     if (stats) stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -339,9 +339,8 @@ void trace_closest_raw(rm_scene* s, const float* rays, uint32_t n, float* t, uin
     ClosestArgs a{};
     a.nodes = s->d_nodes.as<Bvh8Node>();
     a.tris = s->d_tris.as<BvhTri>();
-    a.mode = kTraceQueue;
     a.count = n;
-    a.fetch = &cnt.as<Counters>()->fetch_rays[0];
+    a.fetch = &cnt.as<Counters>()->it[0].fetch_closest;
     a.paths.ox = f[0]; a.paths.oy = f[1]; a.paths.oz = f[2];
     a.paths.dx = f[4]; a.paths.dy = f[5]; a.paths.dz = f[6];
     a.tmin_arr = f[3];
@@ -384,7 +383,7 @@ void trace_any_raw(rm_scene* s, const float* rays, uint32_t n, uint32_t* occlude
     a.nodes = s->d_nodes.as<Bvh8Node>();
     a.tris = s->d_tris.as<BvhTri>();
     a.count = n;
-    a.fetch = &cnt.as<Counters>()->fetch_shadows[0];
+    a.fetch = &cnt.as<Counters>()->it[0].fetch_any;
     a.shadows.ox = f[0]; a.shadows.oy = f[1]; a.shadows.oz = f[2];
     a.shadows.dx = f[4]; a.shadows.dy = f[5]; a.shadows.dz = f[6];
     a.tmin_arr = f[3];

@@ -7,12 +7,10 @@
 
 namespace rm {
 
-__global__ void clear_counters_kernel(Counters* c, uint32_t initial_rays) {
+__global__ void clear_counters_kernel(Counters* c) {
     uint32_t* w = reinterpret_cast<uint32_t*>(c);
     const uint32_t n = (uint32_t)(sizeof(Counters) / 4);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
-    __syncthreads();
-    if (threadIdx.x == 0) c->rays[0] = initial_rays;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0u;
 }
 
 // One quad-tree level: sums the 4 children of each texel in the next finer
@@ -77,8 +75,8 @@ __global__ void accum_f64_kernel(float* img, double* acc, uint32_t n4) {
     img[i] = 0.0f;
 }
 
-hipError_t launch_clear_counters(Counters* c, uint32_t initial_rays, hipStream_t s) {
-    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(256), 0, s, c, initial_rays);
+hipError_t launch_clear_counters(Counters* c, hipStream_t s) {
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(64), dim3(256), 0, s, c);
     return hipGetLastError();
 }
 hipError_t launch_envmap_reduce(float* level, int res, hipStream_t s) {

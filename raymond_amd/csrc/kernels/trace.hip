@@ -6,10 +6,9 @@
 // handleShadowRays (device/kernels/shading/shadow.metal:6-29).
 //
 // Design:
-//  * persistent waves: each 64-lane workgroup repeatedly claims 64 rays with
-//    one atomic on a per-depth fetch counter until the queue (whose length
-//    lives in device memory) is exhausted, so launches need no host
-//    round-trip and lanes of short rays are refilled;
+//  * persistent waves over the device-resident queue length (no host
+//    round-trip): most 64-ray batches are assigned statically, the tail is
+//    claimed dynamically through a per-depth counter (BatchScheduler);
 //  * compressed BVH8 (80-byte nodes, five 16-byte loads, 8-bit child boxes)
 //    with ordered internal-child visits by ray octant;
 //  * traversal stack: the bottom kLdsStack entries per lane in LDS (column
@@ -150,14 +149,39 @@ __device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __res
     return found;
 }
 
-__device__ inline uint32_t wave_fetch(uint32_t* counter) {
-    uint32_t base = 0;
-    if (threadIdx.x == 0) base = atomicAdd(counter, (uint32_t)kTraceBlock);
-    return __shfl(base, 0);
-}
+// Batch scheduler of the persistent waves.  The first 3/4 of the 64-ray
+// batches are assigned statically (wave w takes w, w + nwaves, ...); the rest
+// are claimed one at a time with an atomic on `counter`.  Waves without a
+// batch of their own exit without touching the counter: device-scope atomics
+// on one address serialise across the XCDs, so a queue of a few thousand
+// rays must not cost an atomic per launched wave.
+struct BatchScheduler {
+    uint32_t nb, static_nb, nwaves, b;
+    bool dynamic;
+    __device__ BatchScheduler(uint32_t count) {
+        nb = (count + (uint32_t)kTraceBlock - 1u) / (uint32_t)kTraceBlock;
+        static_nb = nb - nb / 4u;
+        nwaves = gridDim.x;
+        b = blockIdx.x;
+        dynamic = false;
+    }
+    // next batch index, or >= nb when done
+    __device__ uint32_t next(uint32_t* counter) {
+        if (!dynamic && b >= static_nb) dynamic = true;
+        if (dynamic) {
+            uint32_t v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
+            b = static_nb + __shfl(v, 0);
+        }
+        const uint32_t cur = b;
+        if (!dynamic) b += nwaves;
+        return cur;
+    }
+};
 
-// Closest-hit over a queue; mode kTraceCamera also generates the camera rays
-// and writes the initial path state.
+// Closest hit over a path queue.  Camera paths (depth 0) use the camera's
+// clip range, continuation rays (eps, inf) (intersection.metal:201-202);
+// raw batches bring their own ranges.
 template <bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
@@ -166,37 +190,26 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
     StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     unsigned long long sn = 0, st = 0;
+    BatchScheduler sched(count);
+    if (blockIdx.x >= sched.nb) return;
     while (true) {
-        const uint32_t base = wave_fetch(a.fetch);
-        if (base >= count) break;
-        const uint32_t i = base + lane;
+        const uint32_t batch = sched.next(a.fetch);
+        if (batch >= sched.nb) break;
+        const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
         if (i >= count) continue;
-        vec3 o, d;
+        const PathQueue& q = a.paths;
+        const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
+        const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
         float tmin, tmax;
-        if (a.mode == kTraceCamera) {
-            const uint32_t wh = a.width * a.height;
-            const uint32_t s = i / wh;
-            const uint32_t ray_index = i - s * wh;
-            uint32_t x, y;
-            block_linear_to_pixel(ray_index, a.width, a.height, x, y);
-            Prng prng(a.random_seed + s, ray_index);
-            camera_ray(a.camera, x, y, a.width, a.height, prng, o, d);
+        if (a.tmin_arr) {
+            tmin = a.tmin_arr[i];
+            tmax = a.tmax_arr[i];
+        } else if (meta_depth(q.meta[i]) == 0) {
             tmin = a.camera.near_clip;
             tmax = a.camera.far_clip;
-            PathQueue& q = a.paths;
-            q.ox[i] = o.x; q.oy[i] = o.y; q.oz[i] = o.z;
-            q.dx[i] = d.x; q.dy[i] = d.y; q.dz[i] = d.z;
-            q.wx[i] = 1.0f; q.wy[i] = 1.0f; q.wz[i] = 1.0f;
-            q.pdf[i] = kInf;
-            q.seed[i] = prng.seed;
-            q.meta[i] = pack_meta(prng.index, RayCamera, 0);
-            q.pixel[i] = a.image_offset + y * a.width + x;
         } else {
-            const PathQueue& q = a.paths;
-            o = vec3(q.ox[i], q.oy[i], q.oz[i]);
-            d = vec3(q.dx[i], q.dy[i], q.dz[i]);
-            tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
-            tmax = a.tmax_arr ? a.tmax_arr[i] : kInf;
+            tmin = kEps;
+            tmax = kInf;
         }
         TraceResult r;
         traverse<false, STATS>(a.nodes, a.tris, o, d, tmin, tmax, stk, r);
@@ -213,6 +226,45 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
     }
 }
 
+// Path regeneration: appends new camera paths after the survivors of the
+// previous iteration until the queue is full or every sample of the stream
+// has been started.  Path g of the stream is sample s = g / (W H) of the
+// pixel with block-linear index g mod (W H), keyed exactly as the
+// reference's raygen keys (frame seed, rayIndex) (generate.metal:88-121).
+__global__ void regenerate_kernel(RegenArgs a) {
+    const IterCounters& cur = a.counters->it[a.iter];
+    const uint32_t appended = cur.appended;
+    const uint32_t generated = cur.generated;
+    const uint32_t room = a.capacity - appended;
+    const uint32_t left = a.total - generated;
+    const uint32_t k = room < left ? room : left;
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0) {
+        a.counters->it[a.iter].queue = appended + k;
+        a.counters->it[a.iter + 1].generated = generated + k;
+    }
+    const uint32_t wh = a.width * a.height;
+    for (uint32_t t = t0; t < k; t += gridDim.x * blockDim.x) {
+        const uint32_t g = generated + t;
+        const uint32_t s = g / wh;
+        const uint32_t ray_index = g - s * wh;
+        uint32_t x, y;
+        block_linear_to_pixel(ray_index, a.width, a.height, x, y);
+        Prng prng(a.random_seed + s, ray_index);
+        vec3 o, d;
+        camera_ray(a.camera, x, y, a.width, a.height, prng, o, d);
+        const uint32_t j = appended + t;
+        PathQueue& q = a.paths;
+        q.ox[j] = o.x; q.oy[j] = o.y; q.oz[j] = o.z;
+        q.dx[j] = d.x; q.dy[j] = d.y; q.dz[j] = d.z;
+        q.wx[j] = 1.0f; q.wy[j] = 1.0f; q.wz[j] = 1.0f;
+        q.pdf[j] = kInf;
+        q.seed[j] = prng.seed;
+        q.meta[j] = pack_meta(prng.index, RayCamera, 0);
+        q.pixel[j] = a.image_offset + y * a.width + x;
+    }
+}
+
 // Any-hit over the shadow queue.  With accumulate, an unoccluded ray adds
 // (weight, 1) to its pixel (shadow.metal:6-29); otherwise the occlusion flag
 // is written to `occluded`.
@@ -224,10 +276,12 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
     StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     unsigned long long sn = 0, st = 0;
+    BatchScheduler sched(count);
+    if (blockIdx.x >= sched.nb) return;
     while (true) {
-        const uint32_t base = wave_fetch(a.fetch);
-        if (base >= count) break;
-        const uint32_t i = base + lane;
+        const uint32_t batch = sched.next(a.fetch);
+        if (batch >= sched.nb) break;
+        const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
         if (i >= count) continue;
         const ShadowQueue& q = a.shadows;
         const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
@@ -271,6 +325,11 @@ hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hi
 hipError_t launch_trace_any(const AnyArgs& a, int blocks, bool stats, hipStream_t stream) {
     if (stats) hipLaunchKernelGGL(trace_any_kernel<true>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
     else hipLaunchKernelGGL(trace_any_kernel<false>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_regenerate(const RegenArgs& a, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(regenerate_kernel, dim3(blocks), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

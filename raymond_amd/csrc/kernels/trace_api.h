@@ -9,13 +9,9 @@ namespace rm {
 
 constexpr int kTraceBlock = 64;      // one wave64 per workgroup
 constexpr int kMaxTraceStack = 64;   // LDS rows + spill rows; BVH depth is bounded below this
-constexpr int kTraceCamera = 0;
-constexpr int kTraceQueue = 1;
-
 struct ClosestArgs {
     const Bvh8Node* nodes;
     const BvhTri* tris;
-    int mode;
     uint32_t count;             // used when count_ptr is null
     const uint32_t* count_ptr;  // device queue length
     uint32_t* fetch;            // persistent work counter (zero before launch)
@@ -25,7 +21,15 @@ struct ClosestArgs {
     const float* tmax_arr;
     uint2* spill;               // (kMaxTraceStack - 16) x grid threads
     Counters* counters;
-    // camera mode
+    Camera camera;              // clip range of depth-0 paths
+};
+
+struct RegenArgs {
+    Counters* counters;
+    uint32_t iter;
+    uint32_t capacity;          // queue capacity
+    uint32_t total;             // paths in the stream (W H samples)
+    PathQueue paths;            // queue of iteration `iter`
     Camera camera;
     uint32_t width, height, random_seed, image_offset;
 };
@@ -46,6 +50,7 @@ struct AnyArgs {
 
 hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream);
 hipError_t launch_trace_any(const AnyArgs& a, int blocks, bool stats, hipStream_t stream);
+hipError_t launch_regenerate(const RegenArgs& a, int blocks, hipStream_t stream);
 int trace_blocks_per_cu(bool any);
 
 }  // namespace rm

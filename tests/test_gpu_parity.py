@@ -157,3 +157,18 @@ def test_render_spheres_per_sample(small_spheres):
     assert st.total_rays == res.total_rays
     scale = max(np.abs(ref).max(), 1e-6)
     assert np.abs(r.image() - ref).max() <= 1e-4 * scale
+
+
+@pytest.mark.parametrize("spl,samples", [(1, 3), (2, 5)])
+def test_regenerating_stream_matches_frames(cornell, spl, samples):
+    """A stream of `samples` spp through a queue of spl frames (paths are
+    regenerated as others terminate) equals the sum of per-frame oracle renders."""
+    desc, scene, orc = cornell
+    W, H = 64, 64
+    r = api.Renderer(scene, W, H, max_depth=8, samples_per_launch=spl)
+    st = r.execute(samples)
+    ref, res = orc.render(W, H, 8, frames=samples, frame_begin=0)
+    assert st.total_rays == res.total_rays
+    assert st.iterations > 8
+    scale = max(np.abs(ref).max(), 1e-6)
+    assert np.abs(r.image() - ref).max() <= 1e-4 * scale
