@@ -14,6 +14,7 @@
 #define RAYMOND_AMD_RM_HOST_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -27,6 +28,12 @@ int rmh_codegen(const char* materials_json, int noinline, char** out_source, cha
  * ("Base Color" -> "baseColor", MaterialBuilder.swift:19-43). */
 int rmh_identifier(const char* name, char* out, size_t capacity);
 void rmh_free(void* p);
+/* Builds a BVH8 over n triangles (9 floats each) on the host and verifies it:
+ * each triangle referenced once, every quantised child box containing its
+ * subtree.  Returns 0, the node count and the depth; -1 and a message otherwise.
+ * (A test hook for the builder behind rm_scene_build_accel.) */
+int rmh_bvh8_check(const float* tris, uint32_t n, int threads, uint32_t* out_nodes, uint32_t* out_depth, char* err,
+                   size_t err_cap);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,8 @@ EXPORTS = {
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "rmh_free": (None, [C.c_void_p]),
+    "rmh_bvh8_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.c_char_p, C.c_size_t]),
 }
 
 
@@ -174,6 +176,17 @@ def identifier(name):
     buf = C.create_string_buffer(256)
     _check(LIB.rmh_identifier(name.encode(), buf, 256))
     return buf.value.decode()
+
+
+def bvh8_check(tris, threads=0):
+    """Host build + structural check of the BVH8 (rmh_bvh8_check)."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    nodes, depth = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(256)
+    status = LIB.rmh_bvh8_check(tris.ctypes.data, tris.shape[0], threads, C.byref(nodes), C.byref(depth), err, 256)
+    if status != 0:
+        raise RmError(err.value.decode())
+    return nodes.value, depth.value
 
 
 class Device:

@@ -28,7 +28,7 @@ HOST_FLAGS = [
 ]
 
 KERNELS = ["kernels/trace.hip", "kernels/misc.hip"]
-HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/scene.cpp", "host/renderer.cpp", "host/capi.cpp"]
+HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/capi.cpp"]
 
 
 def _run(cmd):

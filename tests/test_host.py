@@ -1,0 +1,210 @@
+"""CPU tests of the host side: codegen, BVH8 builder, light tables, envmap,
+oracle self-consistency and the exported C ABI (no GPU needed)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from oracle import binding
+from tests import devlib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rmh?_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", api.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if " T " in line)
+    for h in ("rm.h", "rm_host.h"):
+        names = declared_functions(h)
+        assert names, h
+        missing = [n for n in names if n not in exported]
+        assert not missing, "%s: not exported: %s" % (h, missing)
+
+
+@pytest.mark.parametrize("name,ident", [
+    ("Base Color", "baseColor"), ("Subsurface IOR", "subsurfaceIor"), ("Value_001", "value_001"),
+    ("Emission Strength", "emissionStrength"), ("BSDF", "bsdf"), ("IOR", "ior"), ("Shader_001", "shader_001"),
+    ("Principled BSDF", "principledBsdf"), ("tall box", "tallBox"), ("Color-Ramp 2", "color_ramp2"),
+])
+def test_identifier_rule(name, ident):
+    # camelCase over space-separated words (MaterialBuilder.swift:19-43)
+    assert api.identifier(name) == ident
+
+
+def graph(nodes):
+    import json
+    return json.dumps({"surface": [{"name": "m", "nodes": nodes}], "light": []})
+
+
+def test_codegen_errors():
+    out = scenes.output_material(("A", "Value"))
+    a = {"type": "MATH", "parameters": {"operation": "ADD"},
+         "inputs": {"Value": scenes._in("VALUE", link=("B", "Value")), "Value_001": scenes._in("VALUE", 1.0)}}
+    b = {"type": "MATH", "parameters": {"operation": "ADD"},
+         "inputs": {"Value": scenes._in("VALUE", link=("A", "Value")), "Value_001": scenes._in("VALUE", 1.0)}}
+    with pytest.raises(api.RmError, match="cycle"):
+        api.codegen(graph({"Out": out, "A": a, "B": b}))
+    multi = {"type": "OUTPUT_MATERIAL", "parameters": {},
+             "inputs": {"Surface": {"type": "SHADER", "links": [["A", "Value"], ["A", "Value"]]}}}
+    with pytest.raises(api.RmError, match="multi-input"):
+        api.codegen(graph({"Out": multi, "A": dict(a, inputs={})}))
+    with pytest.raises(api.RmError, match="unsupported node"):
+        api.codegen(graph({"Out": out, "A": {"type": "TEX_BRICK", "parameters": {}, "inputs": {}}}))
+
+
+def test_codegen_defaults_and_links():
+    src = scenes.cornell_box(8, 8).material_source()
+    # BSDF normal defaults to the shading normal (MaterialBuilder.swift:768-789)
+    assert "n_diffuseBsdf.normal = VECTOR(shading.normal);" in src
+    assert "n_materialOutput.surface = SHADER(n_diffuseBsdf.bsdf);" in src
+    assert "case 4: s_light(ctx, shading); break;" in src
+    src2 = scenes.sphere_field(8, 8, grid=1, level=0).material_source()
+    assert "n_sky.vector = VECTOR(shading.generated);" in src2
+
+
+def test_generated_source_compiles_with_gxx():
+    so = binding.compile_materials(scenes.sphere_field(8, 8, grid=2, level=0).material_source())
+    assert os.path.exists(so)
+
+
+def random_soup(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-10, 10, (n, 1, 3))
+    return (c + rng.normal(scale=0.3, size=(n, 3, 3))).reshape(n, 9).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["soup", "flat", "degenerate", "tiny", "one", "huge_range"])
+def test_bvh8_builder_is_conservative(kind):
+    if kind == "soup":
+        tris = random_soup(20000, 1)
+    elif kind == "flat":  # coplanar, zero-thickness boxes as in the Cornell walls
+        t = random_soup(5000, 2)
+        t[:, 2::3] = 0.5
+        tris = t
+    elif kind == "degenerate":  # zero-area and repeated triangles
+        t = random_soup(3000, 3)
+        t[:1000, 3:6] = t[:1000, 0:3]
+        t[1000:2000] = t[0]
+        tris = t
+    elif kind == "tiny":
+        tris = random_soup(3, 4)
+    elif kind == "one":
+        tris = random_soup(1, 5)
+    else:
+        t = random_soup(4000, 6)
+        t[:100] *= 1e6
+        t[100:200] *= 1e-6
+        tris = t
+    nodes, depth = api.bvh8_check(tris)
+    assert nodes >= 1 and depth >= 1
+
+
+def test_bvh8_builder_cornell_and_spheres():
+    for desc in (scenes.cornell_box(8, 8), scenes.sphere_field(8, 8, grid=3, level=4)):
+        verts, _, _, faces, _, inst = desc.arrays()
+        tris = []
+        for e in inst:
+            m = np.array(e["transform"], np.float32).reshape(4, 4).T
+            f = faces[e["face_offset"]:e["face_offset"] + e["face_count"]].astype(np.int64) + e["vertex_offset"]
+            tris.append((verts[f] @ m[:3, :3].T + m[:3, 3]).reshape(-1, 9))
+        api.bvh8_check(np.concatenate(tris))
+
+
+@pytest.fixture(scope="module")
+def cornell_oracle():
+    desc = scenes.cornell_box(32, 32)
+    desc.envmap_resolution = 64
+    orc = binding.OracleScene(desc)
+    orc.build_envmap(threads=4)
+    return desc, orc
+
+
+def test_light_distribution_matches_face_areas(cornell_oracle):
+    # distribution.m:7-39: CDF of face areas under the instance's inner 3x3
+    desc, orc = cornell_oracle
+    verts, _, _, faces, _, inst = desc.arrays()
+    light = [e for e in inst if e["emissive"]][0]
+    m = np.array(light["transform"], np.float64).reshape(4, 4).T[:3, :3]
+    f = faces[light["face_offset"]:light["face_offset"] + light["face_count"]] + light["vertex_offset"]
+    v = verts[f].astype(np.float64) @ m.T
+    areas = 0.5 * np.linalg.norm(np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1]), axis=1)
+    cdf, emissive = orc.read_light_tables(light["face_count"], 1)
+    assert np.allclose(emissive[0], areas.sum(), rtol=1e-6)
+    assert np.allclose(cdf, np.cumsum(areas) / areas.sum(), rtol=1e-6)
+    assert cdf[-1] == np.float32(1.0)
+
+
+def test_envmap_is_normalised_and_sampling_matches_pdf():
+    desc = scenes.sphere_field(8, 8, grid=1, level=0)
+    desc.envmap_resolution = 128
+    orc = binding.OracleScene(desc)
+    orc.build_envmap(threads=4)
+    pdfs, mip = orc.read_envmap()
+    res = 128
+    # pdf is per solid angle: sum(pdf) * 4pi / N == 1 (normalize.metal:4-11)
+    assert abs(pdfs.astype(np.float64).sum() * 4 * np.pi / pdfs.size - 1.0) < 1e-4
+    rng = np.random.default_rng(3)
+    uv = rng.uniform(0, 1, (200000, 2)).astype(np.float32)
+    d, p = devlib.world_sample(pdfs, mip, res, uv)
+    # the sampled direction's texel pdf equals the pdf lookup (WorldLight.hpp:14-62)
+    lookup = devlib.world_pdf(pdfs, mip, res, d)
+    assert np.mean(lookup == p) > 0.99
+    # histogram of sampled texels follows the pdf
+    sq = devlib.warp(1, d, 2)
+    ix = np.clip((sq * res).astype(int), 0, res - 1)
+    counts = np.bincount(ix[:, 1] * res + ix[:, 0], minlength=res * res).astype(np.float64)
+    expected = pdfs.astype(np.float64) * 4 * np.pi / pdfs.size * uv.shape[0]
+    big = expected > 50
+    assert np.all(np.abs(counts[big] - expected[big]) < 6 * np.sqrt(expected[big]) + 2)
+
+
+def test_oracle_bvh_matches_brute_force():
+    desc = scenes.sphere_field(8, 8, grid=2, level=3)
+    orc = binding.OracleScene(desc)
+    rng = np.random.default_rng(9)
+    n = 20000
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(-2, 2, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = np.inf
+    a = orc.trace_closest(r)
+    b = orc.trace_closest(r, brute_force=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    assert np.array_equal(orc.trace_any(r), orc.trace_any(r, brute_force=True))
+
+
+def test_oracle_counts_and_camera_block_linear(cornell_oracle):
+    desc, orc = cornell_oracle
+    img, res = orc.render(32, 32, 8, frames=1)
+    assert res.rays[0] == 32 * 32
+    assert all(res.rays[d + 1] <= res.rays[d] for d in range(7))
+    assert res.shadow_rays[7] == 0  # last depth: never traced
+    assert np.all(img[..., 3] >= 0) and np.isfinite(img).all()
+
+
+def test_empty_world_furnace():
+    """Every camera ray misses under a constant world: radiance = world colour exactly."""
+    d = scenes.SceneDescription("empty")
+    d.surface_materials = [scenes.diffuse_material("d", (0.5, 0.5, 0.5))]
+    d.light_materials = [scenes.background_world("w", (0.25, 0.5, 1.0), 2.0)]
+    d.world_shader = 0
+    d.envmap_resolution = 32
+    d.camera = dict(transform=np.eye(4), focal=1.0)
+    orc = binding.OracleScene(d)
+    orc.build_envmap(threads=2)
+    img, res = orc.render(16, 16, 4, frames=1)
+    assert np.allclose(img[..., :3], np.array([0.5, 1.0, 2.0], np.float32))
+    assert res.total_rays == 256
