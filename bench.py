@@ -1,11 +1,14 @@
 """Benchmark: Mrays/s of the wavefront path tracer on one MI355X.
 
-Workload (configuration 2): 64 level-6 icospheres displaced by 3-octave value
+Default workload (configuration 2): 64 level-6 icospheres displaced by 3-octave value
 noise plus a ground plane (5,242,882 triangles), Principled materials, an
 analytic sky importance-sampled through a 2048^2 envmap, 1920x1080, path
 depth 12.  One step = one sample per pixel (2,073,600 paths) traced through
 all bounces.  Mrays/s counts closest-hit plus shadow rays traced, the
 reference's own accounting (host/Renderer.swift:190-196,249-253).
+
+`--workload graphs` runs configuration 3 instead: the Cornell-box geometry
+with 32 random node-graph materials, depth 8, shading sorted by material.
 
 Prints one JSON line.  `--gpus` above 1 is rejected: this build runs on one GPU.
 """
@@ -23,6 +26,15 @@ NODE_BYTES, TRI_BYTES = 80, 48
 RAY_READ, HIT_WRITE = 28, 20  # origin + direction + meta word; t, prim, inst, 2 barycentrics
 
 
+def workload_name(graphs, W, H, D, sort):
+    if graphs:
+        return ("shader graphs (config 3): Cornell-box geometry, 32 random 20-60 node Cycles graphs (seed 3) "
+                "assigned per triangle, %dx%d, depth %d, plain tracer, 1 spp per step%s" %
+                (W, H, D, ", shading sorted by material" if sort else ""))
+    return ("sphere field (config 2): 64 noise-displaced level-6 icospheres + ground, 5,242,882 "
+            "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -31,7 +43,13 @@ def main():
     ap.add_argument("--spp-per-launch", type=int, default=1, help="path-queue capacity in frames")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
+    ap.add_argument("--workload", choices=["spheres", "graphs"], default="spheres",
+                    help="spheres: 5.2M-triangle sphere field; graphs: 32 random node-graph materials")
+    ap.add_argument("--material-sort", type=int, default=-1,
+                    help="group hits by material before shading (-1 = on for graphs, off for spheres)")
+    ap.add_argument("--inline-materials", type=int, default=-1,
+                    help="1: material graphs inlined into the shading kernel, 0: out-of-line calls, -1: by size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-pixels", type=int, default=0, help="oracle sample size (0 = full frames)")
@@ -46,11 +64,17 @@ def main():
     import raymond_amd.api as api
     import raymond_amd.scenes as scenes
 
-    W, H, D = args.width, args.height, args.depth
+    W, H = args.width, args.height
+    graphs = args.workload == "graphs"
+    D = args.depth or (8 if graphs else 12)
+    sort = args.material_sort if args.material_sort >= 0 else int(graphs)
     t0 = time.time()
-    desc = scenes.sphere_field(W, H)
+    desc = scenes.shader_graph_scene(W, H) if graphs else scenes.sphere_field(W, H)
     dev = api.Device(0)
-    scene = scenes.build_device_scene(dev, desc)
+    # the graph scene inlines its materials (one compile, cached; faster shading)
+    inline = args.inline_materials if args.inline_materials >= 0 else (1 if graphs else -1)
+    noinline = None if inline < 0 else not inline
+    scene = scenes.build_device_scene(dev, desc, noinline=noinline)
     sst = scene.stats()
     setup_s = time.time() - t0
     print("setup %.1fs: %d triangles, BVH8 %d nodes depth %d (%.1fs), JIT %.1fs, envmap %.2fs" %
@@ -59,7 +83,7 @@ def main():
 
     spl = args.spp_per_launch
     r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
-    r.set_uniforms(timing=1, rr_depth=0)
+    r.set_uniforms(timing=1, rr_depth=0, material_sort=sort)
     if args.warmup:
         r.execute(args.warmup)
     # the timed steps run as one regenerating stream of `steps` samples per pixel
@@ -67,7 +91,7 @@ def main():
     st = r.execute(args.steps)
     wall = time.perf_counter() - t_start
     totals = dict(rays=st.total_rays, closest=st.closest_total, shadow=st.shadow_total,
-                  ms_closest=st.ms_trace_closest, ms_shade=st.ms_shade, ms_any=st.ms_trace_any,
+                  ms_closest=st.ms_trace_closest, ms_shade=st.ms_shade, ms_any=st.ms_trace_any, ms_sort=st.ms_sort,
                   nclosest=st.trace_closest_launches, nany=st.trace_any_launches, iters=st.iterations)
     closest_total = st.closest_total
     mrays = totals["rays"] / wall / 1e6
@@ -82,10 +106,11 @@ def main():
     bytes_per_ray = RAY_READ + HIT_WRITE + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
     closest_bytes = closest_total * bytes_per_ray
     achieved = closest_bytes / (totals["ms_closest"] * 1e-3) / 1e9 if totals["ms_closest"] > 0 else 0.0
-    print("timed %d steps in %.3fs: %.1f Mrays/s, %.1f Msamples/s; closest %.2f ms/step, shade %.2f, any %.2f; "
-          "nodes/ray %.1f tris/ray %.1f" % (args.steps, wall, mrays, samples_per_s / 1e6,
-                                            totals["ms_closest"] / args.steps, totals["ms_shade"] / args.steps,
-                                            totals["ms_any"] / args.steps, nodes_per_ray, tris_per_ray),
+    print("timed %d steps in %.3fs: %.1f Mrays/s, %.1f Msamples/s; closest %.2f ms/step, sort %.2f, shade %.2f, "
+          "any %.2f; nodes/ray %.1f tris/ray %.1f" % (args.steps, wall, mrays, samples_per_s / 1e6,
+                                                      totals["ms_closest"] / args.steps, totals["ms_sort"] / args.steps,
+                                                      totals["ms_shade"] / args.steps, totals["ms_any"] / args.steps,
+                                                      nodes_per_ray, tris_per_ray),
           file=sys.stderr, flush=True)
 
     cpu = None
@@ -124,13 +149,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "sphere field (config 2): 64 noise-displaced level-6 icospheres + ground, 5,242,882 "
-                        "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D),
+            "workload": workload_name(graphs, W, H, D, sort),
             "width": W, "height": H, "max_depth": D, "queue_frames": spl, "triangles": int(sst.triangles),
-            "iterations": int(totals["iters"]),
+            "iterations": int(totals["iters"]), "material_sort": bool(sort),
         },
         "samples_per_s": round(samples_per_s, 1),
         "stage_ms_per_step": {"trace_closest": round(totals["ms_closest"] / args.steps, 3),
+                              "material_sort": round(totals["ms_sort"] / args.steps, 3),
                               "shade": round(totals["ms_shade"] / args.steps, 3),
                               "trace_any_shadow": round(totals["ms_any"] / args.steps, 3)},
         "roofline": {"kernel": "trace_closest", "bound": "hbm", "achieved": round(achieved, 1),

@@ -207,6 +207,7 @@ typedef struct rm_uniforms {
     uint32_t accumulate_f64;  /* also accumulate every launch into an FP64 buffer */
     uint32_t timing;          /* record per-stage GPU times (hipEvents) */
     uint32_t traversal_stats; /* count BVH nodes / triangles visited (slower) */
+    uint32_t material_sort;   /* group each iteration's hits by material before shading */
 } rm_uniforms;
 
 typedef struct rm_render_stats {
@@ -227,6 +228,7 @@ typedef struct rm_render_stats {
     uint32_t trace_any_launches;
     uint64_t iterations;        /* bounce-loop iterations */
     uint64_t nodes_closest, tris_closest, nodes_any, tris_any; /* traversal_stats */
+    double ms_sort;             /* material sort (keys + scatter) when timing != 0 */
 } rm_render_stats;
 
 /* Replaces Renderer.init + setSize: samples_per_launch paths per pixel are

@@ -34,6 +34,15 @@ void rmh_free(void* p);
  * (A test hook for the builder behind rm_scene_build_accel.) */
 int rmh_bvh8_check(const float* tris, uint32_t n, int threads, uint32_t* out_nodes, uint32_t* out_depth, char* err,
                    size_t err_cap);
+/* Compiles generated material source for gfx950 with hiprtc, without a device,
+ * into the code-object cache that rm_scene_compile_materials reads (the
+ * directory in $RM_JIT_CACHE, else ~/.cache/raymond_amd).  Same source and
+ * options give the same cache entry, so a later scene load skips the compile.
+ * Returns 0 (out_hit = 1 if the entry already existed) or RM_ERROR_COMPILE
+ * with the hiprtc log in `log`.  No reference counterpart: the reference
+ * compiles its Metal library at scene load (SceneLoader.swift:62-75). */
+int rmh_jit_precompile(const char* source, size_t length, const char* options, int* out_hit, char* log,
+                       size_t log_capacity);
 
 #ifdef __cplusplus
 }

@@ -85,7 +85,7 @@ class SceneStats(C.Structure):
 class Uniforms(C.Structure):
     _fields_ = [("sampling_mode", C.c_uint32), ("accumulate", C.c_uint32), ("exposure", C.c_float),
                 ("tonemapping", C.c_uint32), ("rr_depth", C.c_uint32), ("accumulate_f64", C.c_uint32),
-                ("timing", C.c_uint32), ("traversal_stats", C.c_uint32)]
+                ("timing", C.c_uint32), ("traversal_stats", C.c_uint32), ("material_sort", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -94,7 +94,7 @@ class RenderStats(C.Structure):
                 ("ms_shade", C.c_double), ("ms_trace_any", C.c_double),
                 ("trace_closest_launches", C.c_uint32), ("trace_any_launches", C.c_uint32),
                 ("iterations", C.c_uint64), ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_any", C.c_uint64),
-                ("tris_any", C.c_uint64)]
+                ("tris_any", C.c_uint64), ("ms_sort", C.c_double)]
 
 
 EXPORTS = {
@@ -130,6 +130,7 @@ EXPORTS = {
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "rmh_free": (None, [C.c_void_p]),
+    "rmh_jit_precompile": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "rmh_bvh8_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.c_char_p, C.c_size_t]),
 }
@@ -176,6 +177,18 @@ def identifier(name):
     buf = C.create_string_buffer(256)
     _check(LIB.rmh_identifier(name.encode(), buf, 256))
     return buf.value.decode()
+
+
+def jit_precompile(source, options=""):
+    """Compiles material source into the on-disk code-object cache without a
+    device (rmh_jit_precompile); returns True if the entry already existed."""
+    b = source.encode()
+    hit = C.c_int(0)
+    log = C.create_string_buffer(1 << 16)
+    status = LIB.rmh_jit_precompile(b, len(b), options.encode(), C.byref(hit), log, len(log))
+    if status != 0:
+        raise RmError("material compile failed:\n%s" % log.value.decode())
+    return bool(hit.value)
 
 
 def bvh8_check(tris, threads=0):

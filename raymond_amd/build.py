@@ -27,7 +27,7 @@ HOST_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
-KERNELS = ["kernels/trace.hip", "kernels/misc.hip"]
+KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip"]
 HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/capi.cpp"]
 
 
@@ -66,5 +66,41 @@ def build(verbose=False):
     return LIB
 
 
+JIT_CACHE = os.path.join(HERE, "jitcache")
+
+
+def warm_jit_cache(verbose=False, big=False):
+    """Compiles the material modules of the scenes the tests, smoke check and
+    default benchmark load into jitcache/ beside the library, which the JIT
+    reads before compiling; `big` adds the 32-graph scene in both forms.
+    hiprtc serialises compilations inside one process, so each module gets a
+    process of its own.  Returns {scene: seconds}."""
+    import json
+    root = os.path.dirname(HERE)
+    sys.path.insert(0, root)
+    from raymond_amd import scenes
+    jobs = scenes.bundled_material_sources(big=big)
+    os.makedirs(JIT_CACHE, exist_ok=True)
+    env = dict(os.environ, RM_JIT_CACHE=JIT_CACHE)
+    code = ("import sys, time; sys.path.insert(0, %r); import raymond_amd.api as api; "
+            "src = sys.stdin.read(); t = time.time(); api.jit_precompile(src); print(round(time.time() - t, 1))" % root)
+    procs = []
+    for name, src in jobs:
+        p = subprocess.Popen([sys.executable, "-c", code], env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                             stderr=subprocess.PIPE, text=True)
+        procs.append((name, src, p))
+    times = {}
+    for name, src, p in procs:
+        out, err = p.communicate(src)
+        if p.returncode != 0:
+            raise RuntimeError("precompiling %s failed:\n%s" % (name, err[-4000:]))
+        times[name] = float(out.strip().splitlines()[-1])
+    if verbose:
+        print("jit cache:", json.dumps(times))
+    return times
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    if "--warm" in sys.argv:
+        warm_jit_cache(verbose=True, big="--big" in sys.argv)

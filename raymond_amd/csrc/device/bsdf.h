@@ -240,7 +240,7 @@ struct Transmission {
         }
         const float eta = frame::cos_theta(wo) > 0.0f ? ior : 1.0f / ior;
         const vec3 wh = is_reflection ? normalize(wi + wo) : normalize(wi * eta + wo);
-        const float alpha = is_reflection ? reflection_alpha : transmission_alpha;
+        const float alpha = select(is_reflection, reflection_alpha, transmission_alpha);
         pdf = (aniso_ggx(wh, alpha, alpha) * aniso_smith_g1(wo, wh, alpha, alpha)) *
               abs_(dot(wo, wh) / frame::cos_theta(wo));
         if (!(pdf > 0.0f)) {
@@ -266,7 +266,7 @@ struct Transmission {
         const float eta = frame::cos_theta(wo) > 0.0f ? ior : 1.0f / ior;
         const float Fr = only_refract ? 0.0f : fresnel_dielectric_cos(frame::cos_theta(wo), eta);
         const bool is_reflection = rnd.x < Fr;
-        const float alpha = is_reflection ? reflection_alpha : transmission_alpha;
+        const float alpha = select(is_reflection, reflection_alpha, transmission_alpha);
         // VNDF sampling consumes the unscaled rnd (lobes/Transmission.hpp:71-84)
         const vec3 wh = sample_ggx_vndf(rnd, alpha, alpha, wo);
         r.pdf = (aniso_ggx(wh, alpha, alpha) * aniso_smith_g1(wo, wh, alpha, alpha)) *

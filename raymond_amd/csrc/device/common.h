@@ -148,6 +148,15 @@ struct vec4 {
     RM_HD vec3 xyz() const { return vec3(x, y, z); }
 };
 
+// Component-wise selects on values (never on addresses).
+RM_HD float select(bool c, float a, float b) { return c ? a : b; }
+RM_HD vec3 select(bool c, vec3 a, vec3 b) {
+    return vec3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+RM_HD vec4 select(bool c, vec4 a, vec4 b) {
+    return vec4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+
 RM_HD vec2 operator+(vec2 a, vec2 b) { return vec2(a.x + b.x, a.y + b.y); }
 RM_HD vec2 operator-(vec2 a, vec2 b) { return vec2(a.x - b.x, a.y - b.y); }
 RM_HD vec2 operator*(vec2 a, vec2 b) { return vec2(a.x * b.x, a.y * b.y); }

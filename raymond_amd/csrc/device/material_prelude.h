@@ -13,8 +13,10 @@
 #include "nodes.h"
 #include "lights.h"
 
+// out-of-line material functions (large graph sets): bounds the shading
+// kernel's code size and compile time
 #if defined(RM_GPU)
-#define RM_MATERIAL_NOINLINE __noinline__
+#define RM_MATERIAL_FN_NOINLINE __host__ __device__ __noinline__
 #else
-#define RM_MATERIAL_NOINLINE __attribute__((noinline))
+#define RM_MATERIAL_FN_NOINLINE __attribute__((noinline))
 #endif

@@ -218,17 +218,29 @@ struct ColorRamp {
     vec4 color;
     Element elements[NumElements];
 
-    // linear interpolation between stops (device/nodes/nodes.hpp:1013-1046)
+    // Linear interpolation between stops (device/nodes/nodes.hpp:1013-1046).
+    // The first stop at or above fac is found by an unrolled select chain, so
+    // the stop array is never indexed at run time and stays in registers.
     RM_HD void compute(const SceneView&, ShadingContext&) {
         if (fac < elements[0].position) { color = elements[0].color; return; }
         if (fac > elements[NumElements - 1].position) { color = elements[NumElements - 1].color; return; }
+        // Selects are taken on values: a conditional between two lvalues
+        // selects their addresses, which would pin the array in memory.
+        bool found = false;
+        float pa = 0.0f, pb = 0.0f;
+        vec4 ca, cb;
+#pragma unroll
         for (int i = 1; i < NumElements; ++i) {
-            if (elements[i].position >= fac) {
-                const Element a = elements[i - 1], b = elements[i];
-                const float v = (fac - a.position) / (b.position - a.position);
-                color = (1.0f - v) * a.color + v * b.color;
-                return;
-            }
+            const bool take = !found && elements[i].position >= fac;
+            pa = select(take, elements[i - 1].position, pa);
+            pb = select(take, elements[i].position, pb);
+            ca = select(take, elements[i - 1].color, ca);
+            cb = select(take, elements[i].color, cb);
+            found = found || take;
+        }
+        if (found) {
+            const float v = (fac - pa) / (pb - pa);
+            color = (1.0f - v) * ca + v * cb;
         }
     }
 };
@@ -261,8 +273,8 @@ struct Math {
         case kMath::OPERATION_DIVIDE: value = safe_divide(value, value_001, 0.0f); break;
         case kMath::OPERATION_MULTIPLY_ADD: value = value * value_001 + value_002; break;
         case kMath::OPERATION_POWER: value = pow_(value, value_001); break;
-        case kMath::OPERATION_MINIMUM: value = value < value_001 ? value : value_001; break;
-        case kMath::OPERATION_MAXIMUM: value = value > value_001 ? value : value_001; break;
+        case kMath::OPERATION_MINIMUM: value = select(value < value_001, value, value_001); break;
+        case kMath::OPERATION_MAXIMUM: value = select(value > value_001, value, value_001); break;
         case kMath::OPERATION_TANGENT: value = tan_(value); break;
         case kMath::OPERATION_LESS_THAN: value = value < value_001 ? 1.0f : 0.0f; break;
         case kMath::OPERATION_GREATER_THAN: value = value > value_001 ? 1.0f : 0.0f; break;

@@ -60,10 +60,17 @@ struct IterCounters {
 };
 
 constexpr int kMaxIters = 2048;
+// Material-sort bins: surface materials 0..kSortBins-3 get their own bin
+// (higher ids share bin kSortBins-2) and misses go to bin kSortBins-1.
+constexpr int kSortBins = 256;
 
 // Device counters of one render stream (zeroed by a kernel at stream start).
 struct Counters {
     IterCounters it[kMaxIters];
+    // material-sort bins, double-buffered by iteration parity: scatter(i)
+    // clears the set that keys(i+1) fills
+    uint32_t sort_bins[2][kSortBins];
+    uint32_t sort_cursor[2][kSortBins];
     // traversal statistics (only written by the statistics variants)
     unsigned long long nodes_closest, tris_closest, nodes_any, tris_any;
 };

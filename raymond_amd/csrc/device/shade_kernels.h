@@ -120,7 +120,7 @@ struct ShadeOut {
     uint32_t pixel = 0;
 };
 
-__device__ inline void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
+__device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
     const SceneView& ctx = a.ctx;
     const uint32_t mode = a.u.sampling_mode;
     const vec3 dir(a.cur.dx[i], a.cur.dy[i], a.cur.dz[i]);
@@ -223,9 +223,9 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeA
     const uint32_t count = a.counters->it[a.iter].queue;
     const uint32_t stride = gridDim.x * kShadeBlock;
     for (uint32_t base = blockIdx.x * kShadeBlock; base < count; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t k = base + threadIdx.x;
         ShadeOut out;
-        if (i < count) shade_path(a, i, out);
+        if (k < count) shade_path(a, a.order ? a.order[k] : k, out);
         uint32_t sslot, nslot;
         block_append2(&a.counters->it[a.iter].shadows, out.shadow, &a.counters->it[a.iter + 1].appended, out.next,
                       sslot, nslot);

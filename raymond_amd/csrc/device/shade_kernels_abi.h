@@ -14,6 +14,7 @@ struct ShadeArgs {
     PathQueue next;
     ShadowQueue shadows;
     Counters* counters;
+    const uint32_t* order;     // queue entries grouped by material, or null
     float* image;          // RGBA accumulation
     FrameUniforms u;
     uint32_t iter;             // iteration of the regenerating bounce loop

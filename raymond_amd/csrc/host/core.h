@@ -119,7 +119,7 @@ struct rm_renderer {
     rm_scene* scene = nullptr;
     uint32_t width = 0, height = 0, max_depth = 8, spl = 1;
     size_t capacity = 0;
-    rm::DevBuf q_buf[2], hit_buf, shadow_buf, counters_buf, image_buf, image64_buf, resolve_buf;
+    rm::DevBuf q_buf[2], hit_buf, shadow_buf, counters_buf, image_buf, image64_buf, resolve_buf, sort_buf;
     rm::PathQueue q[2]{};
     rm::HitQueue hits{};
     rm::ShadowQueue shadows{};

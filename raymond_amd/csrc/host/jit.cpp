@@ -5,6 +5,7 @@
 // headers are embedded in this library and handed to hiprtc as in-memory
 // includes.  Code objects are cached on disk, keyed by a hash of the full
 // translation unit, the options and the header texts.
+#include <dlfcn.h>
 #include <hip/hiprtc.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -35,6 +36,18 @@ static uint64_t fnv1a(const void* data, size_t n, uint64_t h) {
         h *= 1099511628211ull;
     }
     return h;
+}
+
+// `jitcache/` beside libraymond_amd.so: code objects compiled ahead of time
+// (rmh_jit_precompile with RM_JIT_CACHE pointing there) travel with the build.
+static std::string bundled_cache_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&bundled_cache_dir), &info) && info.dli_fname) {
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        if (slash != std::string::npos) return lib.substr(0, slash) + "/jitcache";
+    }
+    return "jitcache";
 }
 
 static std::string cache_dir() {
@@ -75,8 +88,9 @@ bool jit_compile(const std::string& body, const std::string& options, std::vecto
     char name[64];
     snprintf(name, sizeof(name), "%016llx.co", (unsigned long long)h);
     const std::string path = cache_dir() + "/" + name;
-    {
-        std::ifstream f(path, std::ios::binary);
+    // the writable cache first, then a read-only one shipped next to the library
+    for (const std::string& p : {path, bundled_cache_dir() + "/" + name}) {
+        std::ifstream f(p, std::ios::binary);
         if (f) {
             code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
             if (!code.empty()) {
@@ -123,3 +137,20 @@ bool jit_compile(const std::string& body, const std::string& options, std::vecto
 }
 
 }  // namespace rm
+
+extern "C" int rmh_jit_precompile(const char* source, size_t length, const char* options, int* out_hit, char* log,
+                                  size_t log_capacity) {
+    if (out_hit) *out_hit = 0;
+    if (!source) return RM_ERROR_INVALID;
+    std::vector<char> code;
+    std::string text;
+    bool hit = false;
+    const bool ok = rm::jit_compile(std::string(source, length), options ? options : "", code, text, hit);
+    if (log && log_capacity) {
+        const size_t n = text.size() < log_capacity - 1 ? text.size() : log_capacity - 1;
+        memcpy(log, text.data(), n);
+        log[n] = '\0';
+    }
+    if (out_hit) *out_hit = hit ? 1 : 0;
+    return ok ? RM_OK : RM_ERROR_COMPILE;
+}

@@ -132,8 +132,8 @@ class Codegen {
         std::string out;
         out += "// generated material code\n";
         out += "namespace rm {\n\n";
-        const char* q = noinline ? "RM_MATERIAL_NOINLINE " : "";
-        for (const auto& f : functions_) out += std::string("RM_HD ") + q + "void " + f.first + "(const SceneView& ctx, ShadingContext& shading);\n";
+        const std::string q = noinline ? "RM_MATERIAL_FN_NOINLINE " : "RM_HD ";
+        for (const auto& f : functions_) out += q + "void " + f.first + "(const SceneView& ctx, ShadingContext& shading);\n";
         out += "\n";
         for (const auto& t : tables_) {
             out += "RM_HD void " + t.first + "(uint32_t index, const SceneView& ctx, ShadingContext& shading) {\n";
@@ -142,7 +142,7 @@ class Codegen {
                 out += "    case " + std::to_string(i) + ": " + t.second[i] + "(ctx, shading); break;\n";
             out += "    default: break;\n    }\n}\n\n";
         }
-        for (const auto& f : functions_) out += f.second;
+        for (const auto& f : functions_) out += (noinline ? "RM_MATERIAL_FN_NOINLINE" : "RM_HD") + f.second.substr(5);
         out += "}  // namespace rm\n";
         return out;
     }

@@ -6,6 +6,7 @@
 
 #include "../device/shade_kernels_abi.h"
 #include "../kernels/misc_api.h"
+#include "../kernels/sort_api.h"
 #include "../kernels/trace_api.h"
 #include "core.h"
 #include "scene_api.h"
@@ -74,7 +75,7 @@ namespace {
 struct StageTimer {
     bool on;
     hipStream_t st;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, shade, any;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, sort, shade, any;
     std::vector<hipEvent_t>& pool;
     size_t used = 0;
     hipEvent_t get() {
@@ -133,7 +134,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     const int sblocks = cus * shade_bpc;
     const uint32_t total = W * H * samples;
 
-    StageTimer tm{r->u.timing != 0, st, {}, {}, {}, r->events};
+    StageTimer tm{r->u.timing != 0, st, {}, {}, {}, {}, r->events};
     if (!r->u.accumulate) RM_HIP(hipMemsetAsync(image, 0, r->image_buf.bytes, st));
     RM_HIP(launch_clear_counters(cnt, st));
 
@@ -165,6 +166,19 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     ca.counters = cnt;
     ca.camera = s->view.camera;
 
+    const bool sorting = r->u.material_sort != 0;
+    SortArgs so{};
+    if (sorting) {
+        if (!r->sort_buf.ptr) r->sort_buf.alloc(r->capacity * 2 * sizeof(uint32_t));
+        so.hits = r->hits;
+        so.instances = s->view.instances;
+        so.materials = s->view.materials;
+        so.keys = r->sort_buf.as<uint32_t>();
+        so.order = so.keys + r->capacity;
+        so.counters = cnt;
+    }
+    const int sort_blocks = cus * 4;
+
     AnyArgs aa{};
     aa.nodes = ca.nodes;
     aa.tris = ca.tris;
@@ -189,6 +203,13 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             hipEvent_t e = tm.begin();
             RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
             tm.end(e, tm.closest);
+            if (sorting) {
+                so.iter = iter;
+                hipEvent_t eo = tm.begin();
+                RM_HIP(launch_sort_keys(so, sort_blocks, st));
+                RM_HIP(launch_sort_scatter(so, sort_blocks, st));
+                tm.end(eo, tm.sort);
+            }
             ShadeArgs sa;
             sa.ctx = s->view;
             sa.cur = cur;
@@ -196,6 +217,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             sa.next = next;
             sa.shadows = r->shadows;
             sa.counters = cnt;
+            sa.order = sorting ? so.order : nullptr;
             sa.image = image;
             sa.u = fu;
             sa.iter = iter;
@@ -242,6 +264,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         if (tm.on) {
             stats->ms_trace_closest += StageTimer::sum(tm.closest);
             stats->ms_shade += StageTimer::sum(tm.shade);
+            stats->ms_sort += StageTimer::sum(tm.sort);
             stats->ms_trace_any += StageTimer::sum(tm.any);
         }
         stats->nodes_closest += tstat[0];

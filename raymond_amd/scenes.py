@@ -199,7 +199,11 @@ class SceneDescription:
     def materials_json(self):
         return json.dumps({"surface": self.surface_materials, "light": self.light_materials})
 
-    def material_source(self, noinline=False):
+    def material_source(self, noinline=None):
+        """Generated material code; large graph sets are emitted as out-of-line
+        functions to bound the shading kernel's registers and compile time."""
+        if noinline is None:
+            noinline = sum(len(m["nodes"]) for m in self.surface_materials) > 400
         return api.codegen(self.materials_json(), noinline)
 
     def arrays(self):
@@ -452,11 +456,219 @@ def sphere_field(width=1920, height=1080, grid=8, level=6, radius=0.4, amplitude
     return d
 
 
-def build_device_scene(device, desc, threads=0, envmap=True):
+def build_device_scene(device, desc, threads=0, envmap=True, noinline=None):
     """SceneLoader.loadScene: materials (JIT), shapes/entities (upload + BVH), lights (envmap)."""
     scene = api.Scene(device, desc)
-    scene.compile_materials(desc.material_source())
+    scene.compile_materials(desc.material_source(noinline))
     scene.build_accel(threads)
     if envmap:
         scene.build_envmap()
     return scene
+
+
+# ---------------------------------------------------------------------------
+# Config 3: seeded random node graphs
+
+
+def _node(t, inputs, params=None):
+    return {"type": t, "parameters": params or {}, "inputs": inputs}
+
+
+def random_graph_material(name, rng, min_nodes=20, max_nodes=60):
+    """A random Cycles-style graph of 20-60 nodes from {noise, voronoi, 4-stop ramp,
+    math add/mul/pow, mix RGB, bump, mix shader, principled}; the output is a
+    principled BSDF or a mix of two.  Every node is reachable from the output.
+
+    The tail (principled, clamps, mix shader, folds of unused outputs) varies
+    in size, so a draw whose node count leaves [min_nodes, max_nodes] is
+    redrawn from the same generator; the result stays a function of the seed."""
+    for _ in range(64):
+        graph = _random_graph_attempt(name, rng, int(rng.integers(min_nodes, max_nodes + 1)))
+        if min_nodes <= len(graph["nodes"]) <= max_nodes:
+            return graph
+    raise RuntimeError("random_graph_material: no graph within [%d, %d] nodes" % (min_nodes, max_nodes))
+
+
+def _random_graph_attempt(name, rng, target):
+    nodes = {}
+    producers = {"value": [], "color": [], "vector": []}
+    consumed = set()
+    counter = [0]
+
+    def new(prefix):
+        counter[0] += 1
+        return "%s %d" % (prefix, counter[0])
+
+    def pick(kind, allow_const=True):
+        pool = producers[kind]
+        fresh = [p for p in pool if p not in consumed]
+        if fresh and rng.uniform() < 0.8:
+            p = fresh[int(rng.integers(len(fresh)))]
+        elif pool and (not allow_const or rng.uniform() < 0.7):
+            p = pool[int(rng.integers(len(pool)))]
+        else:
+            return None
+        consumed.add(p)
+        return p
+
+    def val_input(kind, const):
+        p = pick(kind)
+        t = {"value": "VALUE", "color": "RGBA", "vector": "VECTOR"}[kind]
+        if p is None:
+            return _in(t, const)
+        return _in(t, const, link=p)
+
+    tc = new("Texture Coordinate")
+    nodes[tc] = _node("TEX_COORD", {})
+    producers["vector"].append((tc, "Generated"))
+    kinds = ["noise", "voronoi", "ramp", "math", "mix", "bump"]
+    weights = np.array([0.25, 0.08, 0.17, 0.25, 0.2, 0.05])
+    def unconsumed():
+        return sum(1 for kind in ("color", "value") for p in producers[kind] if p not in consumed)
+
+    # leave room for the principled/mix/output tail (<= 9 nodes) and one fold per unused output
+    while len(nodes) + unconsumed() + 9 < target:
+        k = kinds[int(rng.choice(len(kinds), p=weights / weights.sum()))]
+        if k == "noise":
+            n = new("Noise Texture")
+            nodes[n] = _node("TEX_NOISE", {
+                "Vector": val_input("vector", [0.0, 0.0, 0.0]), "W": _in("VALUE", 0.0),
+                "Scale": _in("VALUE", float(rng.uniform(1, 12))), "Detail": _in("VALUE", float(rng.uniform(0, 4))),
+                "Roughness": _in("VALUE", float(rng.uniform(0.3, 0.7))),
+                "Distortion": _in("VALUE", float(rng.choice([0.0, rng.uniform(0, 1)])))},
+                {"noise_dimensions": "3D"})
+            producers["value"].append((n, "Fac"))
+            producers["color"].append((n, "Color"))
+        elif k == "voronoi":
+            n = new("Voronoi Texture")
+            nodes[n] = _node("TEX_VORONOI", {
+                "Vector": val_input("vector", [0.0, 0.0, 0.0]), "Scale": _in("VALUE", float(rng.uniform(1, 8))),
+                "Smoothness": _in("VALUE", 1.0), "Exponent": _in("VALUE", 0.5), "Randomness": _in("VALUE", 1.0),
+                "W": _in("VALUE", 0.0)}, {"distance": "EUCLIDEAN", "feature": "F1", "dimensions": "3D"})
+            producers["value"].append((n, "Distance"))
+            producers["color"].append((n, "Color"))
+        elif k == "ramp":
+            n = new("Color Ramp")
+            pos = np.sort(rng.uniform(0, 1, 4))
+            els = [{"position": float(p), "color": [float(x) for x in rng.uniform(0, 1, 3)] + [1.0]} for p in pos]
+            nodes[n] = _node("VALTORGB", {"Fac": val_input("value", 0.5)},
+                             {"color_mode": "RGB", "interpolation": "LINEAR", "elements": els})
+            producers["color"].append((n, "Color"))
+        elif k == "math":
+            n = new("Math")
+            op = ["ADD", "MULTIPLY", "POWER"][int(rng.integers(3))]
+            nodes[n] = _node("MATH", {"Value": val_input("value", float(rng.uniform(0, 1))),
+                                      "Value_001": val_input("value", float(rng.uniform(0.2, 2.0))),
+                                      "Value_002": _in("VALUE", 0.5)},
+                             {"operation": op, "use_clamp": bool(rng.uniform() < 0.5)})
+            producers["value"].append((n, "Value"))
+        elif k == "mix":
+            n = new("Mix RGB")
+            blend = ["MIX", "ADD", "MULTIPLY", "SCREEN", "OVERLAY", "LIGHTEN", "DARKEN"][int(rng.integers(7))]
+            nodes[n] = _node("MIX_RGB", {"Fac": val_input("value", float(rng.uniform(0, 1))),
+                                         "Color1": val_input("color", [float(x) for x in rng.uniform(0, 1, 3)] + [1.0]),
+                                         "Color2": val_input("color", [float(x) for x in rng.uniform(0, 1, 3)] + [1.0])},
+                             {"blend_type": blend, "use_clamp": bool(rng.uniform() < 0.5)})
+            producers["color"].append((n, "Color"))
+        else:
+            n = new("Bump")
+            nodes[n] = _node("BUMP", {"Strength": _in("VALUE", 1.0), "Distance": _in("VALUE", 1.0),
+                                      "Height": val_input("value", 1.0), "Normal": _in("VECTOR", [0.0, 0.0, 0.0])})
+            producers["vector"].append((n, "Normal"))
+
+    def clamp01(src):
+        m = new("Math")
+        nodes[m] = _node("MATH", {"Value": _in("VALUE", 0.5, link=src), "Value_001": _in("VALUE", 0.0),
+                                  "Value_002": _in("VALUE", 0.5)}, {"operation": "ADD", "use_clamp": True})
+        return (m, "Value")
+
+    def principled():
+        p = new("Principled BSDF")
+        base = pick("color", allow_const=False)
+        rough = pick("value", allow_const=False)
+        metal = pick("value") if rng.uniform() < 0.4 else None
+        normal = pick("vector") if producers["vector"] else None
+        kw = dict(Specular=0.5, Roughness=float(rng.uniform(0.1, 0.9)), Metallic=0.0)
+        node = principled_node(**kw)
+        if base:
+            node["inputs"]["Base Color"] = _in("RGBA", [0.8, 0.8, 0.8, 1.0], link=base)
+        if rough:
+            node["inputs"]["Roughness"] = _in("VALUE", 0.5, link=clamp01(rough))
+        if metal:
+            node["inputs"]["Metallic"] = _in("VALUE", 0.0, link=clamp01(metal))
+        if normal and normal[1] == "Normal":
+            node["inputs"]["Normal"] = _in("VECTOR", [0.0, 0.0, 0.0], link=normal)
+        node["inputs"]["Clearcoat"] = _in("VALUE", float(rng.choice([0.0, rng.uniform(0, 1)])))
+        nodes[p] = node
+        return (p, "BSDF")
+
+    shader = principled()
+    if rng.uniform() < 0.5:
+        s2 = principled()
+        mx = new("Mix Shader")
+        nodes[mx] = _node("MIX_SHADER", {"Fac": val_input("value", float(rng.uniform(0.2, 0.8))),
+                                         "Shader": _in("SHADER", link=shader), "Shader_001": _in("SHADER", link=s2)})
+        shader = (mx, "Shader")
+    # fold producers that nothing consumes into the base-colour chain so every node is reachable
+    leftovers = [p for kind in ("color", "value") for p in producers[kind] if p not in consumed]
+    pnode = [k for k in nodes if k.startswith("Principled")][0]
+    for lp in leftovers:
+        mx = new("Mix RGB")
+        cur = nodes[pnode]["inputs"]["Base Color"]
+        inputs = {"Fac": _in("VALUE", float(rng.uniform(0.1, 0.5))), "Color2": _in("RGBA", [0.5, 0.5, 0.5, 1.0], link=lp)}
+        inputs["Color1"] = cur if "links" in cur else _in("RGBA", cur.get("value", [0.8, 0.8, 0.8, 1.0]))
+        inputs["Color1"]["type"] = "RGBA"
+        nodes[mx] = _node("MIX_RGB", inputs, {"blend_type": "MIX", "use_clamp": True})
+        nodes[pnode]["inputs"]["Base Color"] = _in("RGBA", [0.8, 0.8, 0.8, 1.0], link=(mx, "Color"))
+        consumed.add(lp)
+    nodes[new("Material Output")] = output_material(shader)
+    return {"name": name, "nodes": nodes}
+
+
+def shader_graph_scene(width=1920, height=1080, materials=32, seed=3):
+    """Config 3: config-1 geometry, `materials` random node graphs assigned per triangle."""
+    d = cornell_box(width, height)
+    rng = np.random.default_rng(seed)
+    graphs = [random_graph_material("graph %02d" % i, rng) for i in range(materials)]
+    light = d.surface_materials[4]
+    d.surface_materials = graphs + [light]
+    light_index = materials
+    for si, s in enumerate(d.shapes):
+        if si == 3:  # emitter quad keeps its emission material
+            s.materials = np.full(s.faces.shape[0], light_index, np.uint16)
+        else:
+            s.materials = rng.integers(0, materials, s.faces.shape[0]).astype(np.uint16)
+    d.name = "graphs"
+    return d
+
+
+def count_emitted_nodes(source):
+    """Nodes per material function in generated source (for reporting)."""
+    counts = []
+    for body in source.split("(const SceneView& ctx, ShadingContext& shading) {")[1:]:
+        body = body.split("\n}\n")[0]
+        if "switch (index)" not in body:
+            counts.append(body.count(".compute(ctx, shading);"))
+    return counts
+
+
+def bundled_material_sources(big=False):
+    """(name, material source) of the scenes the tests, the smoke check and the
+    default benchmark build, for warming the code-object cache ahead of time."""
+    out = {}
+    for name, desc in [("cornell", cornell_box(64, 64)),
+                       ("small spheres", sphere_field(96, 54, grid=2, level=3)),
+                       ("graphs, 8 materials", shader_graph_scene(48, 48, materials=8, seed=3)),
+                       ("sphere field", sphere_field(1920, 1080))]:
+        out[name] = desc.material_source()
+    if big:
+        g = shader_graph_scene(1920, 1080)
+        out["graphs, out of line"] = g.material_source(True)
+        out["graphs, inlined"] = g.material_source(False)
+    # scenes with equal materials share one module
+    seen, jobs = set(), []
+    for name, src in out.items():
+        if src not in seen:
+            seen.add(src)
+            jobs.append((name, src))
+    return jobs
