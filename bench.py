@@ -35,6 +35,109 @@ def workload_name(graphs, W, H, D, sort):
             "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D))
 
 
+def bench_cache(args):
+    """Configuration 5: the Cornell box at full HD, depth 12, roulette from
+    bounce 3.  A converged FP64 render of the plain tracer is the reference;
+    the plain tracer and the cached tracer (online training on 1 path in 16,
+    cold start) each render `steps` spp, and the RMSE against the reference is
+    recorded at power-of-two sample counts with the GPU time spent so far."""
+    import numpy as np
+    import raymond_amd.api as api
+    import raymond_amd.scenes as scenes
+
+    W, H = args.width, args.height
+    D = args.depth or 12
+    dev = api.Device(0)
+    desc = scenes.cornell_box(W, H)
+    desc.envmap_resolution = 64  # black world
+    scene = scenes.build_device_scene(dev, desc)
+
+    t0 = time.perf_counter()
+    ref_r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=4)
+    ref_r.set_uniforms(rr_depth=3, accumulate_f64=1)
+    st = ref_r.execute(args.ref_spp)
+    ref_s = time.perf_counter() - t0
+    ref = (ref_r.image_f64()[..., :3] / args.ref_spp).astype(np.float64)
+    del ref_r
+    print("reference: %d spp in %.1fs (%.1f Mrays/s)" % (args.ref_spp, ref_s, st.total_rays / ref_s / 1e6),
+          file=sys.stderr, flush=True)
+
+    def run(cached):
+        r = api.Renderer(scene, W, H, max_depth=D)
+        r.set_uniforms(rr_depth=3)
+        cache = None
+        if cached:
+            cache = api.RadianceCache(dev, seed=1)
+            r.set_cache(cache, mode=1)
+        r.execute(1)  # warm-up (first launches, cache training starts)
+        r.reset()
+        if cache is not None:
+            cache.close()
+            cache = api.RadianceCache(dev, seed=1)
+            r.set_cache(cache, mode=1)
+        points, done, spent, rays = [], 0, 0.0, 0
+        k = 1
+        while done < args.steps:
+            n = min(k, args.steps) - done
+            t = time.perf_counter()
+            s = r.execute(n)
+            spent += time.perf_counter() - t
+            rays += s.total_rays
+            done += n
+            img = r.image()[..., :3].astype(np.float64) / done
+            points.append({"spp": done, "seconds": round(spent, 4),
+                           "rmse": float(np.sqrt(np.mean((img - ref) ** 2)))})
+            k *= 2
+        out = {"points": points, "mrays_per_s": round(rays / spent / 1e6, 2),
+               "samples_per_s": round(W * H * done / spent, 1)}
+        if cache is not None:
+            cs = cache.stats()
+            out["cache"] = {"steps": int(cs.steps), "train_samples": int(cs.samples), "queries": int(cs.queries),
+                            "last_loss": round(float(cs.last_loss), 5)}
+        return out
+
+    plain = run(False)
+    print("plain: %s" % plain["points"][-1], file=sys.stderr, flush=True)
+    cached = run(True)
+    print("cache: %s" % cached["points"][-1], file=sys.stderr, flush=True)
+
+    def time_to(points, target):
+        """Seconds until the RMSE first reaches `target` (log-log interpolation)."""
+        prev = None
+        for p in points:
+            if p["rmse"] <= target:
+                if prev is None:
+                    return p["seconds"]
+                a, b = np.log(prev["rmse"]), np.log(p["rmse"])
+                f = (np.log(target) - a) / (b - a) if b != a else 1.0
+                return float(np.exp(np.log(prev["seconds"]) + f * (np.log(p["seconds"]) - np.log(prev["seconds"]))))
+            prev = p
+        return None
+
+    target = plain["points"][-1]["rmse"]
+    t_plain = plain["points"][-1]["seconds"]
+    t_cache = time_to(cached["points"], target)
+    out = {
+        "metric": "Mrays/s + RMSE vs high-spp reference at equal time, cache vs plain",
+        "value": cached["mrays_per_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "radiance cache (config 5): Cornell box %dx%d, depth %d, roulette from bounce 3, "
+                               "hash grid 16x2 FP16 x 2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 "
+                               "trains; reference: plain tracer, %d spp, FP64 accumulation" % (W, H, D, args.ref_spp)},
+        "plain": plain,
+        "cached": cached,
+        "plain_rmse_at_steps": target,
+        "cache_seconds_to_plain_rmse": None if t_cache is None else round(t_cache, 4),
+        "time_ratio_cache_vs_plain": None if t_cache is None else round(t_cache / t_plain, 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,8 +147,10 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
-    ap.add_argument("--workload", choices=["spheres", "graphs"], default="spheres",
-                    help="spheres: 5.2M-triangle sphere field; graphs: 32 random node-graph materials")
+    ap.add_argument("--workload", choices=["spheres", "graphs", "cache"], default="spheres",
+                    help="spheres: 5.2M-triangle sphere field; graphs: 32 random node-graph materials; "
+                         "cache: learned radiance cache against the plain tracer on the Cornell box")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
     ap.add_argument("--material-sort", type=int, default=-1,
                     help="group hits by material before shading (-1 = on for graphs, off for spheres)")
     ap.add_argument("--inline-materials", type=int, default=-1,
@@ -64,6 +169,8 @@ def main():
     import raymond_amd.api as api
     import raymond_amd.scenes as scenes
 
+    if args.workload == "cache":
+        return bench_cache(args)
     W, H = args.width, args.height
     graphs = args.workload == "graphs"
     D = args.depth or (8 if graphs else 12)

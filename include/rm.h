@@ -251,6 +251,67 @@ int rm_read_image_f64(rm_renderer* r, double* rgba, size_t count);
 int rm_resolve(rm_renderer* r, float* rgba, size_t count);
 uint32_t rm_renderer_frame_index(rm_renderer* r);
 
+/* ------------------------------------------------------------------------
+ * Learned radiance cache.  No reference counterpart: the reference traces
+ * every path to termination.  A multiresolution hash-grid encoding (16 levels
+ * x 2 FP16 features, resolutions 16..2048) feeds a 48-64-64-3 ReLU MLP with
+ * FP16 weights; paths end at their first vertex after a diffuse bounce and add
+ * throughput x cache(vertex).  One camera path in 2^train_shift is traced
+ * unbiased instead and its vertices train the cache online (Adam) after every
+ * bounce loop.
+ *
+ * Cache inputs, per vertex (RM_CACHE_FEATURES floats): position mapped to
+ * [0,1]^3 over the scene bounds, then the outgoing direction and the shading
+ * normal as (acos(z)/pi, atan2(y,x)/(2 pi) + 1/2), roughness, albedo RGB and
+ * the specular-lobe probability. */
+#define RM_CACHE_FEATURES 12
+#define RM_CACHE_LEVELS 16
+#define RM_CACHE_MLP_PARAMS 9216 /* W1 [48][64], W2 [64][64], W3 [64][32] (columns 3..31 unused) */
+
+typedef struct rm_cache rm_cache;
+
+typedef struct rm_cache_desc {
+    uint32_t log2_table_size;  /* entries per level = 2^n; default 19 */
+    uint32_t seed;             /* parameter initialisation */
+    float learning_rate;       /* Adam; default 1e-2 */
+    uint32_t train_shift;      /* one camera path in 2^n trains; default 4 */
+    uint32_t batch_size;       /* samples per training step; default 16384 */
+    uint32_t steps_per_launch; /* training steps after each bounce loop; default 4 */
+} rm_cache_desc;
+
+typedef struct rm_cache_stats {
+    uint64_t steps;            /* Adam steps taken */
+    uint64_t samples;          /* training samples consumed */
+    uint64_t queries;          /* cache lookups made by renders */
+    uint64_t train_records;    /* training vertices recorded by renders */
+    float last_loss;           /* mean relative L2 loss of the last step */
+    double ms_infer, ms_train; /* GPU time of lookups and training (timing uniform) */
+} rm_cache_stats;
+
+/* Zeroed fields take the defaults. */
+int rm_cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache** out);
+void rm_cache_destroy(rm_cache* c);
+int rm_cache_get_stats(rm_cache* c, rm_cache_stats* out);
+/* Number of table floats: RM_CACHE_LEVELS * 2^log2_table_size * 2. */
+uint64_t rm_cache_table_floats(rm_cache* c);
+/* FP32 master parameters (the FP16 copies the kernels read are rounded from
+ * these).  Either pointer may be null. */
+int rm_cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table, size_t table_count);
+int rm_cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const float* table, size_t table_count);
+/* Raw batches (host arrays, n x RM_CACHE_FEATURES row-major inputs):
+ * infer writes n x 3 outputs; gradients computes the mean-loss gradients
+ * without updating (grad_table may be null); train takes `steps` Adam steps
+ * on the batch. */
+int rm_cache_infer(rm_cache* c, const float* features, uint32_t n, float* out);
+int rm_cache_gradients(rm_cache* c, const float* features, const float* targets, uint32_t n, float* grad_mlp,
+                       float* grad_table, float* loss);
+int rm_cache_train(rm_cache* c, const float* features, const float* targets, uint32_t n, uint32_t steps,
+                   float* last_loss);
+/* Attaches a cache to a renderer (null detaches).  mode 1: query and train;
+ * mode 2: query only (frozen weights).  The cache must outlive the renderer's
+ * use of it. */
+int rm_renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -164,3 +164,59 @@ class OracleScene:
 
     def __del__(self):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# Radiance cache (rcache_ref.cpp): double-precision forward and gradients.
+
+def _rc_lib():
+    L = lib()
+    if not getattr(L, "_rc_ready", False):
+        L.or_rc_round_half.restype = C.c_float
+        L.or_rc_round_half.argtypes = [C.c_float]
+        L.or_rc_grid.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
+        L.or_rc_index.restype = C.c_uint32
+        L.or_rc_index.argtypes = [C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.or_rc_forward.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.or_rc_gradients.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
+        L._rc_ready = True
+    return L
+
+
+def rc_round_half(x):
+    return np.array([_rc_lib().or_rc_round_half(float(v)) for v in np.ravel(x)], np.float32)
+
+
+def rc_grid(log2_table):
+    res = np.zeros(16, np.uint32)
+    dense = np.zeros(16, np.uint32)
+    _rc_lib().or_rc_grid(log2_table, res.ctypes.data, dense.ctypes.data)
+    return res, dense
+
+
+def rc_index(log2_table, level, x, y, z):
+    return _rc_lib().or_rc_index(log2_table, level, x, y, z)
+
+
+def rc_forward(log2_table, mlp, table, features):
+    mlp = np.ascontiguousarray(mlp, np.float32)
+    table = np.ascontiguousarray(table, np.float32)
+    f = np.ascontiguousarray(features, np.float32).reshape(-1, 12)
+    out = np.zeros((f.shape[0], 3), np.float64)
+    _rc_lib().or_rc_forward(log2_table, mlp.ctypes.data, table.ctypes.data, f.ctypes.data, f.shape[0],
+                            out.ctypes.data)
+    return out
+
+
+def rc_gradients(log2_table, mlp, table, features, targets):
+    mlp = np.ascontiguousarray(mlp, np.float32)
+    table = np.ascontiguousarray(table, np.float32)
+    f = np.ascontiguousarray(features, np.float32).reshape(-1, 12)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    gm = np.zeros(mlp.size, np.float64)
+    gt = np.zeros(table.size, np.float64)
+    loss = np.zeros(1, np.float64)
+    _rc_lib().or_rc_gradients(log2_table, mlp.ctypes.data, table.ctypes.data, f.ctypes.data, t.ctypes.data,
+                              f.shape[0], gm.ctypes.data, gt.ctypes.data, loss.ctypes.data)
+    return gm, gt, float(loss[0])

@@ -15,6 +15,8 @@
  *   envmap build/reduce/normalize  device/kernels/envmap/{build,reduce,normalize}.metal
  *   closest / any hit       exhaustive loop and an independent binary BVH
  *                           (median split) -- not the product's BVH8
+ *   radiance cache          rcache_ref.cpp: encoding, MLP, loss and gradients
+ *                           in double (no reference counterpart)
  * What it shares with the product (pinned separately by property tests in
  * tests/): the device function library under raymond_amd/csrc/device
  * (RNG, warps, BSDF lobes, lights, node library, elementary functions) and

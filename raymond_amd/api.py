@@ -97,6 +97,21 @@ class RenderStats(C.Structure):
                 ("tris_any", C.c_uint64), ("ms_sort", C.c_double)]
 
 
+class CacheDesc(C.Structure):
+    _fields_ = [("log2_table_size", C.c_uint32), ("seed", C.c_uint32), ("learning_rate", C.c_float),
+                ("train_shift", C.c_uint32), ("batch_size", C.c_uint32), ("steps_per_launch", C.c_uint32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("samples", C.c_uint64), ("queries", C.c_uint64),
+                ("train_records", C.c_uint64), ("last_loss", C.c_float), ("ms_infer", C.c_double),
+                ("ms_train", C.c_double)]
+
+
+CACHE_FEATURES = 12
+CACHE_MLP_PARAMS = 9216
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "rm_last_error": (C.c_char_p, []),
@@ -127,6 +142,17 @@ EXPORTS = {
     "rm_read_image_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "rm_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "rm_renderer_frame_index": (C.c_uint32, [C.c_void_p]),
+    "rm_cache_create": (C.c_int, [C.c_void_p, C.POINTER(CacheDesc), C.POINTER(C.c_void_p)]),
+    "rm_cache_destroy": (None, [C.c_void_p]),
+    "rm_cache_get_stats": (C.c_int, [C.c_void_p, C.POINTER(CacheStats)]),
+    "rm_cache_table_floats": (C.c_uint64, [C.c_void_p]),
+    "rm_cache_read_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
+    "rm_cache_write_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
+    "rm_cache_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "rm_cache_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "rm_cache_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "rm_renderer_set_cache": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "rmh_free": (None, [C.c_void_p]),
@@ -337,9 +363,78 @@ class Renderer:
     def frame_index(self):
         return LIB.rm_renderer_frame_index(self.handle)
 
+    def set_cache(self, cache, mode=1):
+        """Attach a RadianceCache (mode 1: query and train, 2: query only) or detach (None)."""
+        self._cache = cache
+        _check(LIB.rm_renderer_set_cache(self.handle, cache.handle if cache else None, mode if cache else 0))
+
     def close(self):
         if self.handle:
             LIB.rm_renderer_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def _rows(a, k):
+    return np.ascontiguousarray(a, np.float32).reshape(-1, k)
+
+
+class RadianceCache:
+    """Learned radiance cache (rm_cache_*): hash-grid encoding + 48-64-64-3 MLP."""
+
+    def __init__(self, device, log2_table_size=19, seed=0, learning_rate=1e-2, train_shift=4, batch_size=16384,
+                 steps_per_launch=4):
+        self.device = device
+        self.desc = CacheDesc(log2_table_size, seed, learning_rate, train_shift, batch_size, steps_per_launch)
+        self.handle = C.c_void_p()
+        _check(LIB.rm_cache_create(device.handle, C.byref(self.desc), C.byref(self.handle)))
+        self.table_floats = int(LIB.rm_cache_table_floats(self.handle))
+
+    def stats(self):
+        s = CacheStats()
+        _check(LIB.rm_cache_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def read_params(self):
+        mlp = np.zeros(CACHE_MLP_PARAMS, np.float32)
+        table = np.zeros(self.table_floats, np.float32)
+        _check(LIB.rm_cache_read_params(self.handle, mlp.ctypes.data, mlp.size, table.ctypes.data, table.size))
+        return mlp, table
+
+    def write_params(self, mlp=None, table=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        _check(LIB.rm_cache_write_params(self.handle, None if m is None else m.ctypes.data, 0 if m is None else m.size,
+                                         None if t is None else t.ctypes.data, 0 if t is None else t.size))
+
+    def infer(self, features):
+        f = _rows(features, CACHE_FEATURES)
+        out = np.zeros((f.shape[0], 3), np.float32)
+        _check(LIB.rm_cache_infer(self.handle, f.ctypes.data, f.shape[0], out.ctypes.data))
+        return out
+
+    def gradients(self, features, targets, table=True):
+        f = _rows(features, CACHE_FEATURES)
+        t = _rows(targets, 3)
+        gm = np.zeros(CACHE_MLP_PARAMS, np.float32)
+        gt = np.zeros(self.table_floats, np.float32) if table else None
+        loss = np.zeros(1, np.float32)
+        _check(LIB.rm_cache_gradients(self.handle, f.ctypes.data, t.ctypes.data, f.shape[0], gm.ctypes.data,
+                                      None if gt is None else gt.ctypes.data, loss.ctypes.data))
+        return gm, gt, float(loss[0])
+
+    def train(self, features, targets, steps=1):
+        f = _rows(features, CACHE_FEATURES)
+        t = _rows(targets, 3)
+        loss = np.zeros(1, np.float32)
+        _check(LIB.rm_cache_train(self.handle, f.ctypes.data, t.ctypes.data, f.shape[0], steps, loss.ctypes.data))
+        return float(loss[0])
+
+    def close(self):
+        if self.handle:
+            LIB.rm_cache_destroy(self.handle)
             self.handle = C.c_void_p()
 
     def __del__(self):

@@ -27,8 +27,8 @@ HOST_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
-KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip"]
-HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/capi.cpp"]
+KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip", "kernels/rcache.hip"]
+HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/rcache.cpp", "host/capi.cpp"]
 
 
 def _run(cmd):

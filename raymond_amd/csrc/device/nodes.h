@@ -229,7 +229,9 @@ struct ColorRamp {
         bool found = false;
         float pa = 0.0f, pb = 0.0f;
         vec4 ca, cb;
+#if defined(RM_GPU)
 #pragma unroll
+#endif
         for (int i = 1; i < NumElements; ++i) {
             const bool take = !found && elements[i].position >= fac;
             pa = select(take, elements[i - 1].position, pa);

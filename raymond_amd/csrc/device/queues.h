@@ -44,6 +44,7 @@ struct ShadowQueue {
     float* tmax;
     float *wx, *wy, *wz;
     uint32_t* pixel;
+    uint32_t* slot;   // training slot of the emitting path (radiance cache), or null
 };
 
 // Per-iteration counters of the regenerating bounce loop.  Iteration i:
@@ -71,6 +72,8 @@ struct Counters {
     // clears the set that keys(i+1) fills
     uint32_t sort_bins[2][kSortBins];
     uint32_t sort_cursor[2][kSortBins];
+    // radiance cache: lookups and training vertices recorded in this stream
+    uint32_t cache_queries, train_records;
     // traversal statistics (only written by the statistics variants)
     unsigned long long nodes_closest, tris_closest, nodes_any, tris_any;
 };

@@ -71,38 +71,50 @@ __device__ inline void image_add(float* image, uint32_t pixel, vec3 c) {
 constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kShadeWaves = kShadeBlock / 64;
 
-// Workgroup-level compaction of two append streams: ballots give each lane's
+// Workgroup-level compaction of N append streams: ballots give each lane's
 // rank inside its wave, LDS prefix sums give the wave offsets, and one atomic
 // per stream and workgroup reserves the slots (slots in lane order).  Must be
 // reached by every thread of the workgroup.
-__device__ inline void block_append2(uint32_t* c0, bool w0, uint32_t* c1, bool w1, uint32_t& s0, uint32_t& s1) {
-    __shared__ uint32_t wave_cnt[2][kShadeWaves];
-    __shared__ uint32_t base[2];
+template <int N>
+__device__ inline void block_append(uint32_t* const (&ctr)[N], const bool (&want)[N], uint32_t (&slot)[N]) {
+    __shared__ uint32_t wave_cnt[N][kShadeWaves];
+    __shared__ uint32_t base[N];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x / 64u;
-    const unsigned long long m0 = __ballot(w0), m1 = __ballot(w1);
     const unsigned long long lt = (1ull << lane) - 1ull;
-    if (lane == 0) {
-        wave_cnt[0][wave] = (uint32_t)__popcll(m0);
-        wave_cnt[1][wave] = (uint32_t)__popcll(m1);
+    unsigned long long m[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        m[k] = __ballot(want[k]);
+        if (lane == 0) wave_cnt[k][wave] = (uint32_t)__popcll(m[k]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t0 = 0, t1 = 0;
+    if (threadIdx.x < (uint32_t)N) {
+        const int k = (int)threadIdx.x;
+        uint32_t t = 0;
         for (uint32_t w = 0; w < kShadeWaves; w++) {
-            const uint32_t a = wave_cnt[0][w], b = wave_cnt[1][w];
-            wave_cnt[0][w] = t0;
-            wave_cnt[1][w] = t1;
-            t0 += a;
-            t1 += b;
+            const uint32_t c = wave_cnt[k][w];
+            wave_cnt[k][w] = t;
+            t += c;
         }
-        base[0] = t0 ? atomicAdd(c0, t0) : 0u;
-        base[1] = t1 ? atomicAdd(c1, t1) : 0u;
+        base[k] = t ? atomicAdd(ctr[k], t) : 0u;
     }
     __syncthreads();
-    s0 = base[0] + wave_cnt[0][wave] + (uint32_t)__popcll(m0 & lt);
-    s1 = base[1] + wave_cnt[1][wave] + (uint32_t)__popcll(m1 & lt);
+#pragma unroll
+    for (int k = 0; k < N; k++) slot[k] = base[k] + wave_cnt[k][wave] + (uint32_t)__popcll(m[k] & lt);
     __syncthreads();
+}
+
+// Radiance-cache inputs of a shaded vertex.
+__device__ inline void cache_features(const CacheView& cv, const ShadingContext& sh, vec3 n, float* f) {
+    const vec3 p = (sh.position - cv.bounds_min) * cv.bounds_inv_extent;
+    f[0] = saturate(p.x);
+    f[1] = saturate(p.y);
+    f[2] = saturate(p.z);
+    const UberShader& m = sh.material;
+    const float spec = m.lobe_prob[1];
+    const float rough = spec > 0.0f ? sqrt_(sqrt_(m.specular.alpha_x * m.specular.alpha_y)) : 1.0f;
+    rc_vertex_features(sh.wo, n, rough, m.albedo(), spec, f + 3);
 }
 
 // One path vertex (device/kernels/shading/intersection.metal:43-208).
@@ -118,6 +130,11 @@ struct ShadeOut {
     uint32_t n_meta = 0;
     uint32_t seed = 0;
     uint32_t pixel = 0;
+    // radiance cache: 1 = lookup (the path ends here), 2 = training vertex
+    uint32_t cache_kind = 0;
+    uint32_t slot = kRcNoSlot;
+    vec3 c_t, c_r;
+    float feat[kRcFeatureFloats];
 };
 
 __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
@@ -135,6 +152,26 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
     const bool collect_emission = isinf_(ray_pdf) || mode != 1u;
     const float hit_t = a.hits.t[i];
 
+    // a training path of the radiance cache also sums its own contributions
+    const CacheView& cv = a.cache;
+    const uint32_t slot = cv.slots ? cv.slots[i] : kRcNoSlot;
+    const bool training = slot != kRcNoSlot;
+    float* slot_rad = training ? cv.slot_radiance + 3 * (size_t)(slot & ~kRcRecorded) : nullptr;
+    vec3 rad(0.0f);
+    if (training) rad = vec3(slot_rad[0], slot_rad[1], slot_rad[2]);
+    out.slot = slot;
+    auto contribute = [&](vec3 c) {
+        image_add(a.image, pixel, c);
+        if (training) rad += c;
+    };
+    auto store_radiance = [&]() {
+        if (training) {
+            slot_rad[0] = rad.x;
+            slot_rad[1] = rad.y;
+            slot_rad[2] = rad.z;
+        }
+    };
+
     ShadingContext shading;
     shading.ray_flags = flags;
     shading.rnd = prng.sample3d();
@@ -144,7 +181,8 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
         if (collect_emission) {
             const float mis = (mode == 0u || isinf_(ray_pdf)) ? 1.0f : mis_weight(ray_pdf, lights_envmap_pdf(ctx, dir));
             evaluate_environment(ctx, shading);
-            image_add(a.image, pixel, (mis * ray_weight) * shading.material.emission);
+            contribute((mis * ray_weight) * shading.material.emission);
+            store_radiance();
         }
         return;
     }
@@ -159,7 +197,7 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
     if (collect_emission && mean(shading.material.emission) != 0.0f) {
         const float mis = (mode == 0u || isinf_(ray_pdf)) ? 1.0f
                                                            : mis_weight(ray_pdf, lights_shape_pdf(ctx, inst, shading));
-        image_add(a.image, pixel, (mis * ray_weight) * shading.material.emission);
+        contribute((mis * ray_weight) * shading.material.emission);
     }
 
     vec3 sh_normal;
@@ -169,6 +207,26 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
         vec3 geo = shading.true_normal;
         if (dot(geo, shading.wo) < 0.0f) geo = geo * -1.0f;
         sh_normal = ensure_valid_reflection(geo, shading.wo, shading.material.normal);
+    }
+
+    // Radiance cache: the first vertex after a diffuse bounce ends an
+    // ordinary path with a lookup; a training path records that vertex (its
+    // target is the radiance the rest of the path gathers) and goes on.
+    if (cv.enabled && depth >= 1u && (flags & RayDiffuse) && shading.material.alpha >= 1.0f &&
+        !shading.material.is_delta()) {
+        if (!training) {
+            out.cache_kind = 1u;
+            out.c_t = ray_weight;
+            cache_features(cv, shading, sh_normal, out.feat);
+            return;
+        }
+        if (!(slot & kRcRecorded)) {
+            out.cache_kind = 2u;
+            out.c_t = ray_weight;
+            out.c_r = rad;
+            cache_features(cv, shading, sh_normal, out.feat);
+            out.slot = slot | kRcRecorded;
+        }
     }
 
     // the last depth's shadow and continuation rays are never traced
@@ -191,9 +249,10 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
             }
         } else {
             // the reference applies the path weight twice here (intersection.metal:155,176)
-            image_add(a.image, pixel, ray_weight * contribution);
+            contribute(ray_weight * contribution);
         }
     }
+    store_radiance();
 
     const BsdfSample s = shading.material.sample(shading.rnd, -dir, sh_normal, shading.true_normal, flags);
     const vec3 w = ray_weight * s.weight;
@@ -226,9 +285,34 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeA
         const uint32_t k = base + threadIdx.x;
         ShadeOut out;
         if (k < count) shade_path(a, a.order ? a.order[k] : k, out);
-        uint32_t sslot, nslot;
-        block_append2(&a.counters->it[a.iter].shadows, out.shadow, &a.counters->it[a.iter + 1].appended, out.next,
-                      sslot, nslot);
+        uint32_t* const ctr[4] = {&a.counters->it[a.iter].shadows, &a.counters->it[a.iter + 1].appended,
+                                  &a.counters->cache_queries, &a.counters->train_records};
+        const bool want[4] = {out.shadow, out.next, out.cache_kind == 1u, out.cache_kind == 2u};
+        uint32_t at[4];
+        block_append<4>(ctr, want, at);
+        const uint32_t sslot = at[0], nslot = at[1];
+        if (out.cache_kind == 1u && at[2] < a.cache.query_capacity) {
+            const RcQueryQueue& q = a.cache.queries;
+            const uint32_t j = at[2];
+            q.pixel[j] = out.pixel;
+            q.tx[j] = out.c_t.x;
+            q.ty[j] = out.c_t.y;
+            q.tz[j] = out.c_t.z;
+#pragma unroll
+            for (int f = 0; f < kRcFeatureFloats; f++) q.feat[(size_t)f * a.cache.query_capacity + j] = out.feat[f];
+        } else if (out.cache_kind == 2u && at[3] < a.cache.train_capacity) {
+            const RcTrainQueue& q = a.cache.train;
+            const uint32_t j = at[3];
+            q.slot[j] = out.slot & ~kRcRecorded;
+            q.tx[j] = out.c_t.x;
+            q.ty[j] = out.c_t.y;
+            q.tz[j] = out.c_t.z;
+            q.rx[j] = out.c_r.x;
+            q.ry[j] = out.c_r.y;
+            q.rz[j] = out.c_r.z;
+#pragma unroll
+            for (int f = 0; f < kRcFeatureFloats; f++) q.feat[(size_t)f * a.cache.train_capacity + j] = out.feat[f];
+        }
         if (out.shadow) {
             a.shadows.ox[sslot] = out.s_origin.x;
             a.shadows.oy[sslot] = out.s_origin.y;
@@ -241,6 +325,7 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeA
             a.shadows.wy[sslot] = out.s_weight.y;
             a.shadows.wz[sslot] = out.s_weight.z;
             a.shadows.pixel[sslot] = out.pixel;
+            if (a.shadows.slot) a.shadows.slot[sslot] = out.slot;
         }
         if (out.next) {
             a.next.ox[nslot] = out.n_origin.x;
@@ -256,6 +341,7 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeA
             a.next.seed[nslot] = out.seed;
             a.next.meta[nslot] = out.n_meta;
             a.next.pixel[nslot] = out.pixel;
+            if (a.cache.slots_next) a.cache.slots_next[nslot] = out.slot;
         }
     }
 }

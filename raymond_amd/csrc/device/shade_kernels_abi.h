@@ -3,6 +3,7 @@
 #pragma once
 #include "common.h"
 #include "queues.h"
+#include "rcache.h"
 #include "scene.h"
 
 namespace rm {
@@ -15,6 +16,7 @@ struct ShadeArgs {
     ShadowQueue shadows;
     Counters* counters;
     const uint32_t* order;     // queue entries grouped by material, or null
+    CacheView cache;           // radiance cache (enabled = 0: off)
     float* image;          // RGBA accumulation
     FrameUniforms u;
     uint32_t iter;             // iteration of the regenerating bounce loop

@@ -5,6 +5,7 @@
 
 #include "../kernels/trace_api.h"
 #include "core.h"
+#include "rcache_host.h"
 #include "scene_api.h"
 
 namespace rm {
@@ -282,5 +283,94 @@ int rm_resolve(rm_renderer* r, float* rgba, size_t count) {
 }
 
 uint32_t rm_renderer_frame_index(rm_renderer* r) { return r ? r->frame_index : 0; }
+
+int rm_cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache** out) {
+    RM_NULL_CHECK(dev);
+    RM_NULL_CHECK(out);
+    *out = nullptr;
+    RM_GUARD_BEGIN
+    rm_cache* c = new rm_cache();
+    try {
+        rm::cache_create(dev, desc, c);
+    } catch (...) {
+        delete c;
+        throw;
+    }
+    *out = c;
+    RM_GUARD_END
+}
+
+void rm_cache_destroy(rm_cache* c) { delete c; }
+
+int rm_cache_get_stats(rm_cache* c, rm_cache_stats* out) {
+    RM_NULL_CHECK(c);
+    RM_NULL_CHECK(out);
+    out->steps = c->steps;
+    out->samples = c->samples;
+    out->queries = c->queries;
+    out->train_records = c->records;
+    out->last_loss = c->last_loss;
+    out->ms_infer = c->ms_infer;
+    out->ms_train = c->ms_train;
+    return RM_OK;
+}
+
+uint64_t rm_cache_table_floats(rm_cache* c) { return c ? (uint64_t)c->table_floats : 0; }
+
+int rm_cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table, size_t table_count) {
+    RM_NULL_CHECK(c);
+    RM_GUARD_BEGIN
+    rm::cache_read_params(c, mlp, mlp_count, table, table_count);
+    RM_GUARD_END
+}
+
+int rm_cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const float* table, size_t table_count) {
+    RM_NULL_CHECK(c);
+    RM_GUARD_BEGIN
+    rm::cache_write_params(c, mlp, mlp_count, table, table_count);
+    RM_GUARD_END
+}
+
+int rm_cache_infer(rm_cache* c, const float* features, uint32_t n, float* out) {
+    RM_NULL_CHECK(c);
+    if (n) {
+        RM_NULL_CHECK(features);
+        RM_NULL_CHECK(out);
+    }
+    RM_GUARD_BEGIN
+    rm::cache_infer_raw(c, features, n, out);
+    RM_GUARD_END
+}
+
+int rm_cache_gradients(rm_cache* c, const float* features, const float* targets, uint32_t n, float* grad_mlp,
+                       float* grad_table, float* loss) {
+    RM_NULL_CHECK(c);
+    RM_NULL_CHECK(features);
+    RM_NULL_CHECK(targets);
+    RM_GUARD_BEGIN
+    rm::cache_gradients_raw(c, features, targets, n, grad_mlp, grad_table, loss);
+    RM_GUARD_END
+}
+
+int rm_cache_train(rm_cache* c, const float* features, const float* targets, uint32_t n, uint32_t steps,
+                   float* last_loss) {
+    RM_NULL_CHECK(c);
+    RM_NULL_CHECK(features);
+    RM_NULL_CHECK(targets);
+    RM_GUARD_BEGIN
+    rm::cache_train_raw(c, features, targets, n, steps, last_loss);
+    RM_GUARD_END
+}
+
+int rm_renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode) {
+    RM_NULL_CHECK(r);
+    if (c && (mode < 1 || mode > 2)) {
+        rm::set_error("cache mode must be 1 (query and train) or 2 (query only)");
+        return RM_ERROR_INVALID;
+    }
+    RM_GUARD_BEGIN
+    rm::renderer_set_cache(r, c, c ? mode : 0u);
+    RM_GUARD_END
+}
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include "../../../include/rm.h"
 #include "../device/bvh.h"
 #include "../device/queues.h"
+#include "../device/rcache.h"
 #include "../device/scene.h"
 
 namespace rm {
@@ -126,4 +127,11 @@ struct rm_renderer {
     rm_uniforms u{};
     uint32_t frame_index = 0;  // samples accumulated so far
     std::vector<hipEvent_t> events;
+    // radiance cache (rm_renderer_set_cache)
+    rm_cache* cache = nullptr;
+    uint32_t cache_mode = 0;           // 0 off, 1 query + train, 2 query only
+    size_t slot_capacity = 0;          // training paths per stream
+    rm::DevBuf slot_buf, slot_rad_buf, query_buf, train_buf, target_buf, shadow_slot_buf;
+    rm::CacheView cv{};
+    rm::vec3 world_lo, world_hi;
 };

@@ -9,6 +9,7 @@
 #include "../kernels/sort_api.h"
 #include "../kernels/trace_api.h"
 #include "core.h"
+#include "rcache_host.h"
 #include "scene_api.h"
 
 namespace rm {
@@ -117,6 +118,73 @@ struct StageTimer {
 // kIterChunk iterations.
 static constexpr uint32_t kIterChunk = 8;
 
+void renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode) {
+    r->cache = c;
+    r->cache_mode = c ? mode : 0u;
+    if (!c) return;
+    const std::vector<BuildTri> tris = world_triangles(r->scene);
+    vec3 lo(kInf), hi(-kInf);
+    for (const BuildTri& t : tris)
+        for (const vec3& p : {t.v0, t.v1, t.v2}) {
+            lo = vec3(min_(lo.x, p.x), min_(lo.y, p.y), min_(lo.z, p.z));
+            hi = vec3(max_(hi.x, p.x), max_(hi.y, p.y), max_(hi.z, p.z));
+        }
+    RM_REQUIRE(!tris.empty(), "the radiance cache needs scene geometry");
+    r->world_lo = lo;
+    r->world_hi = hi;
+}
+
+// Buffers of the radiance cache for a stream of `samples` samples per pixel.
+static void prepare_cache(rm_renderer* r, uint32_t samples) {
+    rm_cache* c = r->cache;
+    CacheView& cv = r->cv;
+    const size_t cap = r->capacity;
+    const bool train = r->cache_mode == 1u;
+    const uint32_t shift = c->desc.train_shift;
+    const size_t total = (size_t)r->width * r->height * samples;
+    const size_t slots = train ? ((total + (1ull << shift) - 1) >> shift) + 1 : 0;
+    if (!r->query_buf.ptr) r->query_buf.alloc(cap * (4 + kRcFeatureFloats) * 4);
+    if (train && slots > r->slot_capacity) {
+        r->slot_capacity = slots;
+        r->slot_rad_buf.alloc(slots * 3 * 4);
+        r->train_buf.alloc(slots * (7 + kRcFeatureFloats) * 4);
+        r->target_buf.alloc(slots * 3 * 4);
+    }
+    if (train && !r->slot_buf.ptr) {
+        r->slot_buf.alloc(cap * 2 * 4);
+        r->shadow_slot_buf.alloc(cap * 4);
+    }
+    cv = CacheView{};
+    cv.enabled = 1u;
+    cv.train_shift = shift;
+    cv.query_capacity = (uint32_t)cap;
+    cv.train_capacity = (uint32_t)r->slot_capacity;
+    cv.slot_capacity = (uint32_t)r->slot_capacity;
+    cv.bounds_min = r->world_lo;
+    const vec3 ext = r->world_hi - r->world_lo;
+    cv.bounds_inv_extent = vec3(1.0f / max_(ext.x, 1e-6f), 1.0f / max_(ext.y, 1e-6f), 1.0f / max_(ext.z, 1e-6f));
+    uint32_t* qu = r->query_buf.as<uint32_t>();
+    float* qf = r->query_buf.as<float>();
+    cv.queries.pixel = qu;
+    cv.queries.tx = qf + cap;
+    cv.queries.ty = qf + 2 * cap;
+    cv.queries.tz = qf + 3 * cap;
+    cv.queries.feat = qf + 4 * cap;
+    if (train) {
+        const size_t sc = r->slot_capacity;
+        float* tf = r->train_buf.as<float>();
+        cv.train.slot = r->train_buf.as<uint32_t>();
+        cv.train.tx = tf + sc;
+        cv.train.ty = tf + 2 * sc;
+        cv.train.tz = tf + 3 * sc;
+        cv.train.rx = tf + 4 * sc;
+        cv.train.ry = tf + 5 * sc;
+        cv.train.rz = tf + 6 * sc;
+        cv.train.feat = tf + 7 * sc;
+        cv.slot_radiance = r->slot_rad_buf.as<float>();
+    }
+}
+
 static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     rm_scene* s = r->scene;
     rm_device* dev = s->dev;
@@ -179,6 +247,18 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     }
     const int sort_blocks = cus * 4;
 
+    const bool cached = r->cache_mode != 0u;
+    const bool cache_train = r->cache_mode == 1u;
+    if (cached) prepare_cache(r, samples);
+    uint32_t* slot_arr[2] = {nullptr, nullptr};
+    if (cache_train) {
+        slot_arr[0] = r->slot_buf.as<uint32_t>();
+        slot_arr[1] = slot_arr[0] + r->capacity;
+        ga.slot_radiance = r->cv.slot_radiance;
+        ga.train_shift = r->cv.train_shift;
+    }
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> t_infer;
+
     AnyArgs aa{};
     aa.nodes = ca.nodes;
     aa.tris = ca.tris;
@@ -196,6 +276,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             PathQueue& next = r->q[(iter + 1) & 1];
             ga.iter = iter;
             ga.paths = cur;
+            ga.slots = slot_arr[iter & 1];
             RM_HIP(launch_regenerate(ga, cus * 4, st));
             ca.count_ptr = &cnt->it[iter].queue;
             ca.fetch = &cnt->it[iter].fetch_closest;
@@ -218,6 +299,13 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             sa.shadows = r->shadows;
             sa.counters = cnt;
             sa.order = sorting ? so.order : nullptr;
+            sa.cache = CacheView{};
+            if (cached) {
+                sa.cache = r->cv;
+                sa.cache.slots = slot_arr[iter & 1];
+                sa.cache.slots_next = slot_arr[(iter + 1) & 1];
+                sa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
+            }
             sa.image = image;
             sa.u = fu;
             sa.iter = iter;
@@ -225,6 +313,10 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             hipEvent_t es = tm.begin();
             RM_HIP(hipModuleLaunchKernel(s->jit.shade, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
             tm.end(es, tm.shade);
+            if (cached) {
+                aa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
+                aa.slot_radiance = cache_train ? r->cv.slot_radiance : nullptr;
+            }
             aa.count_ptr = &cnt->it[iter].shadows;
             aa.fetch = &cnt->it[iter].fetch_any;
             hipEvent_t ea = tm.begin();
@@ -235,6 +327,18 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         RM_HIP(hipMemcpyAsync(&probe, &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));
         RM_HIP(hipStreamSynchronize(st));
         if (probe.generated >= total && probe.appended == 0) break;
+    }
+    if (cached) {
+        // the lookups of the whole stream in one batch
+        RcInferArgs ia{};
+        ia.p = cache_params(r->cache);
+        ia.q = r->cv.queries;
+        ia.count_ptr = &cnt->cache_queries;
+        ia.stride = r->cv.query_capacity;
+        ia.image = image;
+        hipEvent_t ei = tm.begin();
+        RM_HIP(launch_rc_infer(ia, cus * 8, st));
+        tm.end(ei, t_infer);
     }
     if (r->u.accumulate_f64) {
         if (!r->image64_buf.ptr) {
@@ -247,7 +351,39 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     RM_HIP(hipMemcpyAsync(host_it.data(), cnt->it, iter * sizeof(IterCounters), hipMemcpyDeviceToHost, st));
     unsigned long long tstat[4] = {0, 0, 0, 0};
     RM_HIP(hipMemcpyAsync(tstat, &cnt->nodes_closest, sizeof(tstat), hipMemcpyDeviceToHost, st));
+    uint32_t cache_ctr[2] = {0, 0};  // lookups, training vertices
+    RM_HIP(hipMemcpyAsync(cache_ctr, &cnt->cache_queries, 8, hipMemcpyDeviceToHost, st));
     RM_HIP(hipStreamSynchronize(st));
+    if (cached) {
+        rm_cache* c = r->cache;
+        c->queries += cache_ctr[0];
+        if (tm.on) c->ms_infer += StageTimer::sum(t_infer);
+        const uint32_t records = std::min<uint32_t>(cache_ctr[1], r->cv.train_capacity);
+        if (cache_train && records > 0) {
+            // targets of this stream's training vertices, then Adam steps on them
+            std::vector<std::pair<hipEvent_t, hipEvent_t>> t_train;
+            hipEvent_t et = tm.begin();
+            RM_HIP(launch_rc_targets(r->cv.train, r->cv.slot_radiance, &cnt->train_records, r->cv.train_capacity,
+                                     r->target_buf.as<float>(), st));
+            RcSamples smp{};
+            smp.feat = r->cv.train.feat;
+            smp.target = r->target_buf.as<float>();
+            smp.stride = r->cv.train_capacity;
+            smp.count = records;
+            smp.perm_mul = 2147483647u;  // prime, so a permutation of any smaller count
+            const uint32_t batch = std::min<uint32_t>(c->desc.batch_size, records);
+            for (uint32_t k = 0; k < c->desc.steps_per_launch; k++) {
+                smp.first = (uint32_t)(c->steps * 7919u) + k * batch;
+                cache_step(c, smp, batch, false);
+            }
+            tm.end(et, t_train);
+            c->records += records;
+            if (tm.on) {
+                RM_HIP(hipStreamSynchronize(st));
+                c->ms_train += StageTimer::sum(t_train);
+            }
+        }
+    }
     r->frame_index += samples;
     if (stats) {
         for (uint32_t i = 0; i < iter; i++) {
@@ -281,8 +417,9 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     RM_REQUIRE(s->envmap_built, "envmap not built");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const auto t0 = std::chrono::steady_clock::now();
-    // bound a stream so its iteration count stays within kMaxIters
-    const uint32_t max_stream = 512;
+    // bound a stream so its iteration count stays within kMaxIters; a
+    // training cache learns between streams of one queue's worth of samples
+    const uint32_t max_stream = r->cache_mode != 0u ? r->spl : 512;
     uint32_t left = samples;
     while (left > 0) {
         const uint32_t n = left < max_stream ? left : max_stream;

@@ -27,5 +27,6 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
 void renderer_read_image(rm_renderer* r, float* rgba, size_t count);
 void renderer_read_image_f64(rm_renderer* r, double* rgba, size_t count);
 void renderer_resolve(rm_renderer* r, float* rgba, size_t count);
+void renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode);
 
 }  // namespace rm

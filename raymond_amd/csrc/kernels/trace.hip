@@ -21,6 +21,7 @@
 #include "../device/queues.h"
 #include "../device/camera.h"
 #include "../device/scene.h"
+#include "../device/rcache.h"
 #include "trace_api.h"
 
 namespace rm {
@@ -262,6 +263,15 @@ __global__ void regenerate_kernel(RegenArgs a) {
         q.seed[j] = prng.seed;
         q.meta[j] = pack_meta(prng.index, RayCamera, 0);
         q.pixel[j] = a.image_offset + y * a.width + x;
+        if (a.slots) {
+            const uint32_t slot = rc_training_slot(g, a.random_seed, a.train_shift);
+            a.slots[j] = slot;
+            if (slot != kRcNoSlot) {
+                a.slot_radiance[3 * (size_t)slot + 0] = 0.0f;
+                a.slot_radiance[3 * (size_t)slot + 1] = 0.0f;
+                a.slot_radiance[3 * (size_t)slot + 2] = 0.0f;
+            }
+        }
     }
 }
 
@@ -297,6 +307,14 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
                 atomicAdd(px + 1, q.wy[i]);
                 atomicAdd(px + 2, q.wz[i]);
                 atomicAdd(px + 3, 1.0f);
+                // a training path has one shadow ray in flight, so this is its only writer
+                const uint32_t slot = q.slot ? q.slot[i] : kRcNoSlot;
+                if (slot != kRcNoSlot && a.slot_radiance) {
+                    float* sr = a.slot_radiance + 3 * (size_t)(slot & ~kRcRecorded);
+                    sr[0] += q.wx[i];
+                    sr[1] += q.wy[i];
+                    sr[2] += q.wz[i];
+                }
             }
         } else {
             a.occluded[i] = hit ? 1u : 0u;

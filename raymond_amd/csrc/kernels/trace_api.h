@@ -32,6 +32,10 @@ struct RegenArgs {
     PathQueue paths;            // queue of iteration `iter`
     Camera camera;
     uint32_t width, height, random_seed, image_offset;
+    // radiance cache: training slot per new path (null when the cache is off)
+    uint32_t* slots;
+    float* slot_radiance;
+    uint32_t train_shift;
 };
 
 struct AnyArgs {
@@ -44,6 +48,7 @@ struct AnyArgs {
     const float* tmin_arr;      // optional; else eps
     float* image;               // accumulate unoccluded weights when non-null
     uint32_t* occluded;         // else write 0/1 per ray
+    float* slot_radiance;       // radiance cache: unoccluded weights of training paths
     uint2* spill;
     Counters* counters;
 };
