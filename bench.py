@@ -35,6 +35,66 @@ def workload_name(graphs, W, H, D, sort):
             "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D))
 
 
+def bench_batch(args):
+    """Configuration 4: 64 Cornell-box jobs (seed 4) with 1k-2M triangles,
+    256^2-1024^2 pixels, 16-256 spp and depth 6-12, rendered as one batch: one
+    regenerating bounce loop over all jobs' samples, queue depth and material
+    sort picked by the autotuner.  Reports aggregate Mrays/s and each job's
+    completion time (its makespan from the batch start)."""
+    import numpy as np
+    import raymond_amd.api as api
+    import raymond_amd.scenes as scenes
+
+    t0 = time.time()
+    specs = scenes.batch_jobs(args.jobs, seed=4)
+    dev = api.Device(0)
+    built = [scenes.build_device_scene(dev, d) for d, *_ in specs]
+    batch = api.Batch(dev, [(s, w, h, spp, depth, 0) for s, (_, w, h, spp, depth) in zip(built, specs)])
+    setup_s = time.time() - t0
+    tris = sum(d.num_triangles for d, *_ in specs)
+    paths = sum(w * h * spp for _, w, h, spp, _ in specs)
+    print("setup %.1fs: %d jobs, %d triangles, %.2f G samples" % (setup_s, len(specs), tris, paths / 1e9),
+          file=sys.stderr, flush=True)
+    t1 = time.time()
+    cfg, tried, hit = batch.autotune(rr_depth=3)
+    print("autotune %.1fs (cache %s): queue %d, sort %d; tried %s" % (time.time() - t1, hit, cfg.queue_paths,
+                                                                      cfg.material_sort, tried),
+          file=sys.stderr, flush=True)
+    st = batch.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort, rr_depth=3, timing=0)
+    secs = st.ms_total / 1e3
+    mrays = st.total_rays / secs / 1e6
+    times = batch.job_times()
+    work = np.array([w * h * spp for _, w, h, spp, _ in specs], np.float64)
+    print("batch: %.2fs, %.1f Mrays/s, %d loops, makespan %.1f ms" % (secs, mrays, st.loops, st.makespan_ms),
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "Mrays/s + RMSE vs high-spp reference at equal time, cache vs plain",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven jobs (config 4): %d Cornell-box jobs (seed 4), %d triangles in "
+                               "total (1k-2M each), 256^2-1024^2 pixels, 16-256 spp, depth 6-12, roulette from "
+                               "bounce 3, one regenerating bounce loop" % (len(specs), tris),
+                   "jobs": len(specs), "triangles": tris, "samples": paths,
+                   "queue_paths": cfg.queue_paths, "material_sort": bool(cfg.material_sort),
+                   "autotune": {"cache_hit": hit, "candidates": [
+                       {"queue_paths": q, "material_sort": m, "mrays_per_s": round(r, 1)} for q, m, r in tried]}},
+        "seconds": round(secs, 3),
+        "samples_per_s": round(st.samples / secs, 1),
+        "bounce_loops": st.loops,
+        "makespan_ms": round(st.makespan_ms, 1),
+        "job_completion_ms": {"min": round(float(times.min()), 1), "median": round(float(np.median(times)), 1),
+                              "max": round(float(times.max()), 1)},
+        "jobs": [{"pixels": int(w * h), "spp": spp, "depth": depth, "triangles": d.num_triangles,
+                  "done_ms": round(float(t), 1)} for (d, w, h, spp, depth), t in zip(specs, times)],
+        "setup_s": round(setup_s, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_cache(args):
     """Configuration 5: the Cornell box at full HD, depth 12, roulette from
     bounce 3.  A converged FP64 render of the plain tracer is the reference;
@@ -147,9 +207,11 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
-    ap.add_argument("--workload", choices=["spheres", "graphs", "cache"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "graphs", "cache", "batch"], default="spheres",
                     help="spheres: 5.2M-triangle sphere field; graphs: 32 random node-graph materials; "
-                         "cache: learned radiance cache against the plain tracer on the Cornell box")
+                         "cache: learned radiance cache against the plain tracer on the Cornell box; "
+                         "batch: 64 uneven Cornell-box jobs in one bounce loop")
+    ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
     ap.add_argument("--material-sort", type=int, default=-1,
                     help="group hits by material before shading (-1 = on for graphs, off for spheres)")
@@ -171,6 +233,8 @@ def main():
 
     if args.workload == "cache":
         return bench_cache(args)
+    if args.workload == "batch":
+        return bench_batch(args)
     W, H = args.width, args.height
     graphs = args.workload == "graphs"
     D = args.depth or (8 if graphs else 12)

@@ -307,6 +307,61 @@ int rm_cache_gradients(rm_cache* c, const float* features, const float* targets,
                        float* grad_table, float* loss);
 int rm_cache_train(rm_cache* c, const float* features, const float* targets, uint32_t n, uint32_t steps,
                    float* last_loss);
+/* ------------------------------------------------------------------------
+ * Batches.  No reference counterpart: the reference renders one scene per
+ * Renderer.  Many jobs (scene, resolution, samples, depth) render together in
+ * one regenerating bounce loop: each job's BVH is a subtree of one merged
+ * BVH8 entered at the job's root, each job keeps its lights and camera, and
+ * the paths of all jobs share the queues, the kernels and (optionally) the
+ * material sort.  The jobs' scenes must be fully built, share one material
+ * library and outlive the batch. */
+typedef struct rm_batch rm_batch;
+
+typedef struct rm_job_desc {
+    rm_scene* scene;     /* camera, geometry, lights and envmap of the job */
+    uint32_t width, height;
+    uint32_t samples;    /* samples per pixel */
+    uint32_t max_depth;
+    uint32_t seed;       /* random seed of sample 0 (sample s uses seed + s); 0 = 1 */
+} rm_job_desc;
+
+typedef struct rm_batch_config {
+    uint32_t queue_paths;    /* paths in flight; 0 = 2^21 */
+    uint32_t material_sort;  /* group each iteration's hits by material */
+    uint32_t rr_depth;       /* Russian roulette from this depth (0 = from the first bounce) */
+    uint32_t timing;         /* per-stage GPU times */
+} rm_batch_config;
+
+typedef struct rm_batch_stats {
+    uint64_t total_rays, closest_rays, shadow_rays, samples, iterations;
+    uint32_t loops;          /* bounce loops the batch was split into */
+    uint32_t queue_paths;
+    double ms_total;         /* wall time of the render */
+    double makespan_ms;      /* completion time of the last job */
+    double ms_trace_closest, ms_shade, ms_trace_any, ms_sort;
+} rm_batch_stats;
+
+typedef struct rm_batch_tune_entry {
+    uint32_t queue_paths, material_sort;
+    double mrays_per_s;
+} rm_batch_tune_entry;
+
+int rm_batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch** out);
+void rm_batch_destroy(rm_batch* b);
+int rm_batch_render(rm_batch* b, const rm_batch_config* config, rm_batch_stats* stats);
+/* Completion time of each job in the last render, milliseconds from its start. */
+int rm_batch_job_times(rm_batch* b, double* ms, uint32_t n);
+/* Accumulated RGBA sums of one job (width*height*4 floats; divide by samples). */
+int rm_batch_read_image(rm_batch* b, uint32_t job, float* rgba, size_t count);
+/* Picks queue depth and material sort for this batch by timing a prefix of
+ * it (the first samples of every job) under each candidate, or takes the
+ * choice recorded for the same device and batch signature in the tuner cache
+ * ($RM_TUNE_CACHE, else ~/.cache/raymond_amd/batch_tune.txt).  `best` enters
+ * with the other settings, which are kept.  entries/count report the
+ * candidates timed (none on a cache hit). */
+int rm_batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* entries, uint32_t capacity,
+                      uint32_t* count, int* cache_hit);
+
 /* Attaches a cache to a renderer (null detaches).  mode 1: query and train;
  * mode 2: query only (frozen weights).  The cache must outlive the renderer's
  * use of it. */

@@ -108,6 +108,28 @@ class CacheStats(C.Structure):
                 ("ms_train", C.c_double)]
 
 
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("max_depth", C.c_uint32), ("seed", C.c_uint32)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("timing", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("samples", C.c_uint64), ("iterations", C.c_uint64), ("loops", C.c_uint32),
+                ("queue_paths", C.c_uint32), ("ms_total", C.c_double), ("makespan_ms", C.c_double),
+                ("ms_trace_closest", C.c_double), ("ms_shade", C.c_double), ("ms_trace_any", C.c_double),
+                ("ms_sort", C.c_double)]
+
+
+class BatchTuneEntry(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("mrays_per_s", C.c_double)]
+
+
 CACHE_FEATURES = 12
 CACHE_MLP_PARAMS = 9216
 
@@ -153,6 +175,13 @@ EXPORTS = {
                                      C.c_void_p]),
     "rm_cache_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "rm_renderer_set_cache": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "rm_batch_create": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "rm_batch_destroy": (None, [C.c_void_p]),
+    "rm_batch_render": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(BatchStats)]),
+    "rm_batch_job_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "rm_batch_read_image": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "rm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(BatchTuneEntry), C.c_uint32,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "rmh_free": (None, [C.c_void_p]),
@@ -435,6 +464,54 @@ class RadianceCache:
     def close(self):
         if self.handle:
             LIB.rm_cache_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Many jobs in one bounce loop (rm_batch_*).  jobs: [(Scene, width, height, samples, max_depth, seed)]."""
+
+    def __init__(self, device, jobs):
+        self.device = device
+        self.scenes = [j[0] for j in jobs]  # kept alive: the batch points at their envmaps
+        self.sizes = [(j[1], j[2], j[3]) for j in jobs]
+        arr = (JobDesc * len(jobs))()
+        for k, (scene, w, h, spp, depth, seed) in enumerate(jobs):
+            arr[k] = JobDesc(scene.handle.value, w, h, spp, depth, seed)
+        self.handle = C.c_void_p()
+        _check(LIB.rm_batch_create(device.handle, arr, len(jobs), C.byref(self.handle)))
+
+    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0):
+        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing)
+        st = BatchStats()
+        _check(LIB.rm_batch_render(self.handle, C.byref(cfg), C.byref(st)))
+        return st
+
+    def job_times(self):
+        out = np.zeros(len(self.sizes), np.float64)
+        _check(LIB.rm_batch_job_times(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def image(self, job):
+        w, h, _ = self.sizes[job]
+        out = np.zeros((h, w, 4), np.float32)
+        _check(LIB.rm_batch_read_image(self.handle, job, out.ctypes.data, out.size))
+        return out
+
+    def autotune(self, rr_depth=0):
+        cfg = BatchConfig(0, 0, rr_depth, 0)
+        entries = (BatchTuneEntry * 16)()
+        count = C.c_uint32(0)
+        hit = C.c_int(0)
+        _check(LIB.rm_batch_autotune(self.handle, C.byref(cfg), entries, 16, C.byref(count), C.byref(hit)))
+        tried = [(e.queue_paths, e.material_sort, e.mrays_per_s) for e in entries[:count.value]]
+        return cfg, tried, bool(hit.value)
+
+    def close(self):
+        if self.handle:
+            LIB.rm_batch_destroy(self.handle)
             self.handle = C.c_void_p()
 
     def __del__(self):

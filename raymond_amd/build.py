@@ -28,7 +28,8 @@ HOST_FLAGS = [
 ]
 
 KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip", "kernels/rcache.hip"]
-HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/rcache.cpp", "host/capi.cpp"]
+HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/rcache.cpp", "host/batch.cpp",
+        "host/capi.cpp"]
 
 
 def _run(cmd):
@@ -83,11 +84,14 @@ def warm_jit_cache(verbose=False, big=False):
     os.makedirs(JIT_CACHE, exist_ok=True)
     env = dict(os.environ, RM_JIT_CACHE=JIT_CACHE)
     code = ("import sys, time; sys.path.insert(0, %r); import raymond_amd.api as api; "
-            "src = sys.stdin.read(); t = time.time(); api.jit_precompile(src); print(round(time.time() - t, 1))" % root)
+            "src = sys.stdin.read(); t = time.time(); api.jit_precompile(src, sys.argv[1]); "
+            "print(round(time.time() - t, 1))" % root)
+    # batches compile their library again with the batch shading kernel
+    jobs = [(name, src, "") for name, src in jobs] + [(jobs[0][0] + ", batch kernel", jobs[0][1], "-DRM_BATCH_KERNEL")]
     procs = []
-    for name, src in jobs:
-        p = subprocess.Popen([sys.executable, "-c", code], env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
-                             stderr=subprocess.PIPE, text=True)
+    for name, src, options in jobs:
+        p = subprocess.Popen([sys.executable, "-c", code, options], env=env, stdin=subprocess.PIPE,
+                             stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
         procs.append((name, src, p))
     times = {}
     for name, src, p in procs:

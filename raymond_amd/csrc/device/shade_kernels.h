@@ -133,12 +133,22 @@ struct ShadeOut {
     // radiance cache: 1 = lookup (the path ends here), 2 = training vertex
     uint32_t cache_kind = 0;
     uint32_t slot = kRcNoSlot;
+    uint32_t job = 0;
     vec3 c_t, c_r;
     float feat[kRcFeatureFloats];
 };
 
+template <bool BATCHED>
 __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
-    const SceneView& ctx = a.ctx;
+    // batched jobs: the path's job supplies the lights and the depth limit
+    SceneView job_ctx = a.ctx;
+    uint32_t max_depth = a.u.max_depth;
+    if (BATCHED) {
+        out.job = job_of_pixel(a.batch.jobs, a.batch.num_jobs, a.cur.pixel[i]);
+        job_ctx.lights = a.batch.lights[out.job];
+        max_depth = a.batch.jobs[out.job].max_depth;
+    }
+    const SceneView& ctx = job_ctx;
     const uint32_t mode = a.u.sampling_mode;
     const vec3 dir(a.cur.dx[i], a.cur.dy[i], a.cur.dz[i]);
     const vec3 ray_weight(a.cur.wx[i], a.cur.wy[i], a.cur.wz[i]);
@@ -231,7 +241,7 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
 
     // the last depth's shadow and continuation rays are never traced
     // (Renderer.swift:623-625), so they are not emitted
-    const bool last = depth + 1u >= a.u.max_depth;
+    const bool last = depth + 1u >= max_depth;
     if (mode != 0u) {
         const LightSample nee = lights_sample(ctx, shading, prng);
         float bsdf_pdf;
@@ -278,13 +288,28 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
     }
 }
 
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) {
+// Batched jobs: ended paths per job, counted in LDS and added to the job
+// counters once per workgroup (jobs beyond the LDS table count directly).
+constexpr uint32_t kJobsLds = 256;
+
+template <bool BATCHED>
+__device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArgs& a) {
+    __shared__ uint32_t job_ended[kJobsLds];
+    constexpr bool batched = BATCHED;
+    if (batched) {
+        for (uint32_t j = threadIdx.x; j < kJobsLds; j += kShadeBlock) job_ended[j] = 0u;
+        __syncthreads();
+    }
     const uint32_t count = a.counters->it[a.iter].queue;
     const uint32_t stride = gridDim.x * kShadeBlock;
     for (uint32_t base = blockIdx.x * kShadeBlock; base < count; base += stride) {
         const uint32_t k = base + threadIdx.x;
         ShadeOut out;
-        if (k < count) shade_path(a, a.order ? a.order[k] : k, out);
+        if (k < count) shade_path<BATCHED>(a, a.order ? a.order[k] : k, out);
+        if (batched && k < count && !out.next) {
+            if (out.job < kJobsLds) atomicAdd(&job_ended[out.job], 1u);
+            else atomicAdd(&a.batch.done[out.job], 1u);
+        }
         uint32_t* const ctr[4] = {&a.counters->it[a.iter].shadows, &a.counters->it[a.iter + 1].appended,
                                   &a.counters->cache_queries, &a.counters->train_records};
         const bool want[4] = {out.shadow, out.next, out.cache_kind == 1u, out.cache_kind == 2u};
@@ -344,7 +369,20 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeA
             if (a.cache.slots_next) a.cache.slots_next[nslot] = out.slot;
         }
     }
+    if (batched) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < kJobsLds && j < a.batch.num_jobs; j += kShadeBlock)
+            if (job_ended[j]) atomicAdd(&a.batch.done[j], job_ended[j]);
+    }
 }
+
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) { shade_queue<false>(a); }
+
+#if defined(RM_BATCH_KERNEL)
+// Batched jobs (a.batch.jobs non-null): per-job lights, depth and path counts.
+// Compiled only into the modules of batches.
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_batch_kernel(ShadeArgs a) { shade_queue<true>(a); }
+#endif
 
 // 64 world-shader samples per texel; writes the finest mip level in 2x2-quad
 // order and the unnormalised pdf (device/kernels/envmap/build.metal:4-51).

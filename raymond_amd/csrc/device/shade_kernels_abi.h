@@ -4,6 +4,7 @@
 #include "common.h"
 #include "queues.h"
 #include "rcache.h"
+#include "batch.h"
 #include "scene.h"
 
 namespace rm {
@@ -17,6 +18,7 @@ struct ShadeArgs {
     Counters* counters;
     const uint32_t* order;     // queue entries grouped by material, or null
     CacheView cache;           // radiance cache (enabled = 0: off)
+    BatchView batch;           // batched jobs (jobs null: one scene)
     float* image;          // RGBA accumulation
     FrameUniforms u;
     uint32_t iter;             // iteration of the regenerating bounce loop

@@ -5,6 +5,7 @@
 
 #include "../kernels/trace_api.h"
 #include "core.h"
+#include "batch_api.h"
 #include "rcache_host.h"
 #include "scene_api.h"
 
@@ -359,6 +360,61 @@ int rm_cache_train(rm_cache* c, const float* features, const float* targets, uin
     RM_NULL_CHECK(targets);
     RM_GUARD_BEGIN
     rm::cache_train_raw(c, features, targets, n, steps, last_loss);
+    RM_GUARD_END
+}
+
+int rm_batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch** out) {
+    RM_NULL_CHECK(dev);
+    RM_NULL_CHECK(jobs);
+    RM_NULL_CHECK(out);
+    *out = nullptr;
+    RM_GUARD_BEGIN
+    rm_batch* b = new rm_batch();
+    try {
+        rm::batch_create(dev, jobs, n, b);
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    *out = b;
+    RM_GUARD_END
+}
+
+void rm_batch_destroy(rm_batch* b) {
+    if (!b) return;
+    if (b->renderer)
+        for (hipEvent_t e : b->renderer->events) (void)hipEventDestroy(e);
+    delete b;
+}
+
+int rm_batch_render(rm_batch* b, const rm_batch_config* config, rm_batch_stats* stats) {
+    RM_NULL_CHECK(b);
+    RM_GUARD_BEGIN
+    rm::batch_render(b, config, stats);
+    RM_GUARD_END
+}
+
+int rm_batch_job_times(rm_batch* b, double* ms, uint32_t n) {
+    RM_NULL_CHECK(b);
+    RM_NULL_CHECK(ms);
+    for (uint32_t j = 0; j < n && j < b->job_ms.size(); j++) ms[j] = b->job_ms[j];
+    return RM_OK;
+}
+
+int rm_batch_read_image(rm_batch* b, uint32_t job, float* rgba, size_t count) {
+    RM_NULL_CHECK(b);
+    RM_NULL_CHECK(rgba);
+    RM_GUARD_BEGIN
+    rm::batch_read_image(b, job, rgba, count);
+    RM_GUARD_END
+}
+
+int rm_batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* entries, uint32_t capacity,
+                      uint32_t* count, int* cache_hit) {
+    RM_NULL_CHECK(b);
+    RM_NULL_CHECK(best);
+    RM_GUARD_BEGIN
+    rm::batch_autotune(b, best, entries, capacity, count, cache_hit);
     RM_GUARD_END
 }
 

@@ -10,6 +10,7 @@
 #include "../device/bvh.h"
 #include "../device/queues.h"
 #include "../device/rcache.h"
+#include "../device/batch.h"
 #include "../device/scene.h"
 
 namespace rm {
@@ -63,10 +64,23 @@ struct DevBuf {
 struct JitModule {
     hipModule_t module = nullptr;
     hipFunction_t shade = nullptr;
+    hipFunction_t shade_batch = nullptr;  // present in modules compiled with -DRM_BATCH_KERNEL
     hipFunction_t envmap_build = nullptr;
     ~JitModule() {
         if (module) (void)hipModuleUnload(module);
     }
+};
+
+// One bounce loop of a batch (rm_batch_render): the slices it covers and the
+// per-job completion bookkeeping the host updates at every poll.
+struct BatchRun {
+    const BatchSlice* slices = nullptr;  // device, firsts relative to this loop
+    uint32_t num_slices = 0;
+    uint32_t total = 0;                  // paths of this loop
+    BatchView view{};
+    const std::vector<uint64_t>* job_paths = nullptr;  // paths per job (whole batch)
+    std::vector<double>* job_ms = nullptr;             // completion time per job, < 0 while running
+    double t0_ms = 0.0;                                // steady-clock origin of the batch
 };
 
 // Compiles material source + the shading kernels; caches code objects on disk.
@@ -113,6 +127,7 @@ struct rm_scene {
     bool materials_built = false;
     bool envmap_built = false;
     rm::JitModule jit;
+    std::string material_source;  // as compiled (batches recompile it with the batch kernel)
     rm_scene_stats stats{};
 };
 
@@ -134,4 +149,5 @@ struct rm_renderer {
     rm::DevBuf slot_buf, slot_rad_buf, query_buf, train_buf, target_buf, shadow_slot_buf;
     rm::CacheView cv{};
     rm::vec3 world_lo, world_hi;
+    rm::BatchRun* batch = nullptr;     // set while a batch renders through this renderer
 };

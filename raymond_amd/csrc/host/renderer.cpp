@@ -26,16 +26,22 @@ static void carve_path_queue(DevBuf& buf, size_t cap, PathQueue& q) {
 }
 
 void renderer_create(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, uint32_t spl, rm_renderer* r) {
-    RM_REQUIRE(w > 0 && h > 0, "image size must be positive");
-    RM_REQUIRE(max_depth >= 1 && max_depth <= 255, "max_depth out of range");
     RM_REQUIRE(spl >= 1, "samples_per_launch must be >= 1");
     RM_REQUIRE((uint64_t)w * h * spl < (1ull << 31), "wavefront too large");
+    renderer_init(s, w, h, max_depth, (size_t)w * h * spl, r);
+    r->spl = spl;
+}
+
+void renderer_init(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, size_t capacity, rm_renderer* r) {
+    RM_REQUIRE(w > 0 && h > 0, "image size must be positive");
+    RM_REQUIRE(max_depth >= 1 && max_depth <= 255, "max_depth out of range");
+    RM_REQUIRE(capacity >= 1 && capacity < (1ull << 31), "path queue capacity out of range");
     r->scene = s;
     r->width = w;
     r->height = h;
     r->max_depth = max_depth;
-    r->spl = spl;
-    r->capacity = (size_t)w * h * spl;
+    r->spl = 1;
+    r->capacity = capacity;
     const size_t cap = r->capacity;
     carve_path_queue(r->q_buf[0], cap, r->q[0]);
     carve_path_queue(r->q_buf[1], cap, r->q[1]);
@@ -185,6 +191,18 @@ static void prepare_cache(rm_renderer* r, uint32_t samples) {
     }
 }
 
+// Records the completion time of every job whose paths have all ended.
+static void batch_poll(BatchRun* br, hipStream_t st) {
+    const uint32_t n = br->view.num_jobs;
+    std::vector<uint32_t> done(n);
+    RM_HIP(hipMemcpyAsync(done.data(), br->view.done, n * 4, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+    const double now =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    for (uint32_t j = 0; j < n; j++)
+        if ((*br->job_ms)[j] < 0.0 && done[j] >= (*br->job_paths)[j]) (*br->job_ms)[j] = now - br->t0_ms;
+}
+
 static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     rm_scene* s = r->scene;
     rm_device* dev = s->dev;
@@ -196,11 +214,14 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     const int cus = dev->props.multiProcessorCount;
     const int tblocks = cus * dev->trace_blocks_per_cu;
     const int ablocks = cus * dev->any_blocks_per_cu;
+    BatchRun* br = r->batch;
+    hipFunction_t shade_fn = br ? s->jit.shade_batch : s->jit.shade;
+    RM_REQUIRE(shade_fn != nullptr, "material module lacks the batch shading kernel");
     int shade_bpc = 0;
-    RM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&shade_bpc, s->jit.shade, 256, 0));
+    RM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&shade_bpc, shade_fn, 256, 0));
     if (shade_bpc < 1) shade_bpc = 1;
     const int sblocks = cus * shade_bpc;
-    const uint32_t total = W * H * samples;
+    const uint32_t total = br ? br->total : W * H * samples;
 
     StageTimer tm{r->u.timing != 0, st, {}, {}, {}, {}, r->events};
     if (!r->u.accumulate) RM_HIP(hipMemsetAsync(image, 0, r->image_buf.bytes, st));
@@ -225,6 +246,11 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     ga.height = H;
     ga.random_seed = fu.random_seed;
     ga.image_offset = 0;
+    if (br) {
+        ga.slices = br->slices;
+        ga.num_slices = br->num_slices;
+        ga.jobs = br->view.jobs;
+    }
 
     ClosestArgs ca{};
     ca.nodes = s->d_nodes.as<Bvh8Node>();
@@ -233,6 +259,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     ca.spill = s->d_spill.as<uint2>();
     ca.counters = cnt;
     ca.camera = s->view.camera;
+    if (br) ca.batch = br->view;
 
     const bool sorting = r->u.material_sort != 0;
     SortArgs so{};
@@ -266,6 +293,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     aa.image = image;
     aa.spill = ca.spill;
     aa.counters = cnt;
+    if (br) aa.batch = br->view;
 
     uint32_t iter = 0;
     std::vector<IterCounters> host_it;
@@ -300,6 +328,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             sa.counters = cnt;
             sa.order = sorting ? so.order : nullptr;
             sa.cache = CacheView{};
+            sa.batch = br ? br->view : BatchView{};
             if (cached) {
                 sa.cache = r->cv;
                 sa.cache.slots = slot_arr[iter & 1];
@@ -311,7 +340,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             sa.iter = iter;
             void* args[] = {&sa};
             hipEvent_t es = tm.begin();
-            RM_HIP(hipModuleLaunchKernel(s->jit.shade, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
+            RM_HIP(hipModuleLaunchKernel(shade_fn, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
             tm.end(es, tm.shade);
             if (cached) {
                 aa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
@@ -326,6 +355,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         IterCounters probe{};
         RM_HIP(hipMemcpyAsync(&probe, &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));
         RM_HIP(hipStreamSynchronize(st));
+        if (br) batch_poll(br, st);
         if (probe.generated >= total && probe.appended == 0) break;
     }
     if (cached) {
@@ -408,6 +438,18 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         stats->nodes_any += tstat[2];
         stats->tris_any += tstat[3];
     }
+}
+
+void renderer_render_batch_loop(rm_renderer* r, BatchRun* br, rm_render_stats* stats) {
+    r->batch = br;
+    try {
+        render_stream(r, 1, stats);
+    } catch (...) {
+        r->batch = nullptr;
+        throw;
+    }
+    r->batch = nullptr;
+    batch_poll(br, r->scene->dev->stream);
 }
 
 void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {

@@ -275,6 +275,11 @@ void scene_compile_materials(rm_scene* s, const char* src, size_t len, const cha
     RM_HIP(hipModuleLoadData(&j.module, code.data()));
     RM_HIP(hipModuleGetFunction(&j.shade, j.module, "rm_shade_kernel"));
     RM_HIP(hipModuleGetFunction(&j.envmap_build, j.module, "rm_envmap_build_kernel"));
+    if (hipModuleGetFunction(&j.shade_batch, j.module, "rm_shade_batch_kernel") != hipSuccess) {
+        (void)hipGetLastError();
+        j.shade_batch = nullptr;
+    }
+    s->material_source.assign(src, len);
     s->materials_built = true;
     s->stats.jit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

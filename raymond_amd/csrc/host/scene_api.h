@@ -22,6 +22,10 @@ void trace_closest_raw(rm_scene* s, const float* rays, uint32_t n, float* t, uin
 void trace_any_raw(rm_scene* s, const float* rays, uint32_t n, uint32_t* occluded);
 
 void renderer_create(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, uint32_t spl, rm_renderer* r);
+// A renderer with an explicit path-queue capacity (batches).
+void renderer_init(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, size_t capacity, rm_renderer* r);
+// One bounce loop over the slices of `br` (rm_batch_render).
+void renderer_render_batch_loop(rm_renderer* r, BatchRun* br, rm_render_stats* stats);
 void renderer_reset(rm_renderer* r);
 void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
 void renderer_read_image(rm_renderer* r, float* rgba, size_t count);

@@ -22,6 +22,7 @@
 #include "../device/camera.h"
 #include "../device/scene.h"
 #include "../device/rcache.h"
+#include "../device/batch.h"
 #include "trace_api.h"
 
 namespace rm {
@@ -55,12 +56,13 @@ struct TraceResult {
 
 // Returns true on a hit. ANY: stop at the first hit in (tmin, tmax).
 template <bool ANY, bool STATS>
-__device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, vec3 o, vec3 d,
-                         float tmin, float tmax, const StackRef& stk, TraceResult& res) {
+__device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, uint32_t root, vec3 o,
+                         vec3 d, float tmin, float tmax, const StackRef& stk, TraceResult& res) {
     const vec3 inv = safe_inv_dir(d);
     const RayShear sh = make_shear(d);
     const int oct = dir_octant(d);
-    uint2 cur = make_uint2(0u, (1u << (24 + oct)) | 1u);
+    // a one-child pseudo node whose only child is the root
+    uint2 cur = make_uint2(root, (1u << (24 + oct)) | 1u);
     int sp = 0;
     float best_t = tmax;
     uint32_t best_inst = 0xffffffffu, best_prim = 0xffffffffu;
@@ -201,19 +203,27 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
         const PathQueue& q = a.paths;
         const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
         const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
+        uint32_t root = 0;
+        float near_clip = a.camera.near_clip, far_clip = a.camera.far_clip;
+        if (a.batch.jobs) {
+            const JobInfo& job = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])];
+            root = job.root;
+            near_clip = job.camera.near_clip;
+            far_clip = job.camera.far_clip;
+        }
         float tmin, tmax;
         if (a.tmin_arr) {
             tmin = a.tmin_arr[i];
             tmax = a.tmax_arr[i];
         } else if (meta_depth(q.meta[i]) == 0) {
-            tmin = a.camera.near_clip;
-            tmax = a.camera.far_clip;
+            tmin = near_clip;
+            tmax = far_clip;
         } else {
             tmin = kEps;
             tmax = kInf;
         }
         TraceResult r;
-        traverse<false, STATS>(a.nodes, a.tris, o, d, tmin, tmax, stk, r);
+        traverse<false, STATS>(a.nodes, a.tris, root, o, d, tmin, tmax, stk, r);
         a.hits.t[i] = r.t;
         a.hits.prim[i] = r.prim;
         a.hits.inst[i] = r.inst;
@@ -247,13 +257,28 @@ __global__ void regenerate_kernel(RegenArgs a) {
     const uint32_t wh = a.width * a.height;
     for (uint32_t t = t0; t < k; t += gridDim.x * blockDim.x) {
         const uint32_t g = generated + t;
-        const uint32_t s = g / wh;
-        const uint32_t ray_index = g - s * wh;
+        uint32_t s, ray_index, width = a.width, height = a.height, seed = a.random_seed, image_offset = a.image_offset;
+        Camera cam = a.camera;
+        if (a.slices) {
+            // batched jobs: the slice (job, sample) holding path g
+            const BatchSlice sl = a.slices[slice_of_path(a.slices, a.num_slices, g)];
+            const JobInfo& job = a.jobs[sl.job];
+            s = sl.sample;
+            ray_index = g - sl.first;
+            width = job.width;
+            height = job.height;
+            seed = job.seed;
+            image_offset = job.image_offset;
+            cam = job.camera;
+        } else {
+            s = g / wh;
+            ray_index = g - s * wh;
+        }
         uint32_t x, y;
-        block_linear_to_pixel(ray_index, a.width, a.height, x, y);
-        Prng prng(a.random_seed + s, ray_index);
+        block_linear_to_pixel(ray_index, width, height, x, y);
+        Prng prng(seed + s, ray_index);
         vec3 o, d;
-        camera_ray(a.camera, x, y, a.width, a.height, prng, o, d);
+        camera_ray(cam, x, y, width, height, prng, o, d);
         const uint32_t j = appended + t;
         PathQueue& q = a.paths;
         q.ox[j] = o.x; q.oy[j] = o.y; q.oz[j] = o.z;
@@ -262,7 +287,7 @@ __global__ void regenerate_kernel(RegenArgs a) {
         q.pdf[j] = kInf;
         q.seed[j] = prng.seed;
         q.meta[j] = pack_meta(prng.index, RayCamera, 0);
-        q.pixel[j] = a.image_offset + y * a.width + x;
+        q.pixel[j] = image_offset + y * width + x;
         if (a.slots) {
             const uint32_t slot = rc_training_slot(g, a.random_seed, a.train_shift);
             a.slots[j] = slot;
@@ -298,8 +323,10 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
         const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
         const float tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
         const float tmax = q.tmax[i];
+        uint32_t root = 0;
+        if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
         TraceResult r;
-        const bool hit = traverse<true, STATS>(a.nodes, a.tris, o, d, tmin, tmax, stk, r);
+        const bool hit = traverse<true, STATS>(a.nodes, a.tris, root, o, d, tmin, tmax, stk, r);
         if (a.image) {
             if (!hit) {
                 float* px = a.image + 4u * (size_t)q.pixel[i];

@@ -4,6 +4,7 @@
 #include "../device/bvh.h"
 #include "../device/queues.h"
 #include "../device/scene.h"
+#include "../device/batch.h"
 
 namespace rm {
 
@@ -22,6 +23,7 @@ struct ClosestArgs {
     uint2* spill;               // (kMaxTraceStack - 16) x grid threads
     Counters* counters;
     Camera camera;              // clip range of depth-0 paths
+    BatchView batch;            // batched jobs (jobs null: one scene, root 0)
 };
 
 struct RegenArgs {
@@ -36,6 +38,10 @@ struct RegenArgs {
     uint32_t* slots;
     float* slot_radiance;
     uint32_t train_shift;
+    // batched jobs: path g belongs to the slice containing it (null: one job)
+    const BatchSlice* slices;
+    uint32_t num_slices;
+    const JobInfo* jobs;
 };
 
 struct AnyArgs {
@@ -51,6 +57,7 @@ struct AnyArgs {
     float* slot_radiance;       // radiance cache: unoccluded weights of training paths
     uint2* spill;
     Counters* counters;
+    BatchView batch;
 };
 
 hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream);

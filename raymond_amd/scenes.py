@@ -299,6 +299,26 @@ def merge(parts):
     return Shape(np.concatenate(vs), np.concatenate(ns), np.concatenate(fs), ms, np.concatenate(uvs))
 
 
+def grid_quad(p0, p1, p2, p3, material, k):
+    """The quad p0 p1 p2 p3 as a k x k grid of quads (2 k^2 triangles), flat normal."""
+    if k <= 1:
+        return quad(p0, p1, p2, p3, material)
+    P = np.array([p0, p1, p2, p3], np.float64)
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    n /= np.linalg.norm(n)
+    t = np.linspace(0.0, 1.0, k + 1)
+    u, v = np.meshgrid(t, t, indexing="xy")
+    u, v = u.ravel(), v.ravel()
+    w = np.stack([(1 - u) * (1 - v), u * (1 - v), u * v, (1 - u) * v], 1)
+    verts = w @ P
+    idx = np.arange((k + 1) * (k + 1)).reshape(k + 1, k + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, 1:].ravel(), idx[1:, :-1].ravel()
+    faces = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    uv = np.stack([u, v], 1).astype(np.float32)
+    return verts, np.repeat(n[None], verts.shape[0], 0), faces, [material] * faces.shape[0], uv
+
+
 def unit_cube(material):
     """Cube [-0.5, 0.5]^3 with outward normals, 12 triangles."""
     h = 0.5
@@ -347,6 +367,55 @@ def cornell_box(width=512, height=512):
     d.camera = dict(transform=look_at((0.5, -1.35, 0.5), (0.5, 0.5, 0.5)), near=1e-3, far=1e4, focal=2.4)
     d.width, d.height = width, height
     return d
+
+
+def tessellated_cornell_box(width, height, triangles):
+    """The config-1 Cornell box with walls and boxes tessellated so that the
+    scene has about `triangles` triangles (the emitter stays one quad)."""
+    d = cornell_box(width, height)
+    k = max(1, int(round((triangles / 34.0) ** 0.5)))
+    WHITE, RED, GREEN, TALL = 0, 1, 2, 3
+    d.shapes[0] = merge([
+        grid_quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), WHITE, k),
+        grid_quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), WHITE, k),
+        grid_quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), WHITE, k),
+        grid_quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), RED, k),
+        grid_quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), GREEN, k),
+    ])
+    h = 0.5
+
+    def cube(material):
+        P = lambda x, y, z: (x * h, y * h, z * h)
+        return merge([
+            grid_quad(P(-1, -1, -1), P(-1, 1, -1), P(1, 1, -1), P(1, -1, -1), material, k),
+            grid_quad(P(-1, -1, 1), P(1, -1, 1), P(1, 1, 1), P(-1, 1, 1), material, k),
+            grid_quad(P(-1, -1, -1), P(1, -1, -1), P(1, -1, 1), P(-1, -1, 1), material, k),
+            grid_quad(P(1, 1, -1), P(-1, 1, -1), P(-1, 1, 1), P(1, 1, 1), material, k),
+            grid_quad(P(-1, 1, -1), P(-1, -1, -1), P(-1, -1, 1), P(-1, 1, 1), material, k),
+            grid_quad(P(1, -1, -1), P(1, 1, -1), P(1, 1, 1), P(1, -1, 1), material, k),
+        ])
+
+    d.shapes[1] = cube(WHITE)
+    d.shapes[2] = cube(TALL)
+    d.name = "cornell-%d" % d.num_triangles
+    return d
+
+
+def batch_jobs(n=64, seed=4):
+    """Config 4: n Cornell-box jobs with triangle counts log-uniform in
+    [1k, 2M], resolutions from {256, 512, 768, 1024}^2, 16-256 spp and depth
+    6-12.  Returns [(scene description, width, height, samples, depth)]."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n):
+        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+        res = int(rng.choice([256, 512, 768, 1024]))
+        spp = int(rng.integers(16, 257))
+        depth = int(rng.integers(6, 13))
+        d = tessellated_cornell_box(res, res, tris)
+        d.envmap_resolution = 64  # black world
+        jobs.append((d, res, res, spp, depth))
+    return jobs
 
 
 def icosphere(level):

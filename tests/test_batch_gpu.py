@@ -1,0 +1,82 @@
+"""Batched jobs on the GPU: every job of a batch renders what it renders alone
+(same seeds, same ray counts; images equal up to the order of the
+floating-point accumulation), with and without the material sort; per-job
+completion is recorded; the autotuner times its candidates and caches the
+choice."""
+import os
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+
+pytestmark = pytest.mark.gpu
+
+JOBS = [  # (triangles, width, height, samples, depth)
+    (0, 48, 32, 3, 5),
+    (2000, 32, 32, 5, 8),
+    (300, 24, 40, 2, 12),
+]
+
+
+@pytest.fixture(scope="module")
+def jobs(device):
+    out = []
+    for tris, w, h, spp, depth in JOBS:
+        d = scenes.cornell_box(w, h) if tris == 0 else scenes.tessellated_cornell_box(w, h, tris)
+        d.envmap_resolution = 64
+        out.append((scenes.build_device_scene(device, d), w, h, spp, depth))
+    return out
+
+
+def solo(scene, w, h, spp, depth):
+    r = api.Renderer(scene, w, h, max_depth=depth)
+    st = r.execute(spp)
+    return r.image(), st.total_rays
+
+
+@pytest.mark.parametrize("sort", [0, 1])
+def test_batch_matches_solo_renders(device, jobs, sort):
+    b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
+    st = b.render(queue_paths=4096, material_sort=sort)
+    total = 0
+    for j, (s, w, h, spp, depth) in enumerate(jobs):
+        ref, rays = solo(s, w, h, spp, depth)
+        total += rays
+        img = b.image(j)
+        scale = max(np.abs(ref).max(), 1e-6)
+        assert np.abs(img - ref).max() <= 1e-5 * scale, j
+    assert st.total_rays == total
+    t = b.job_times()
+    assert np.all(t >= 0.0) and st.makespan_ms == pytest.approx(t.max())
+
+
+def test_batch_splits_into_bounce_loops(device, jobs):
+    """A queue much smaller than the batch forces several bounce loops; the
+    result is the same."""
+    b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
+    st = b.render(queue_paths=64)
+    assert st.loops > 1
+    s, w, h, spp, depth = jobs[1]
+    ref, _ = solo(s, w, h, spp, depth)
+    assert np.abs(b.image(1) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-6)
+
+
+def test_autotune_times_candidates_and_caches(device, jobs, tmp_path):
+    os.environ["RM_TUNE_CACHE"] = str(tmp_path / "tune.txt")
+    try:
+        b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
+        cfg, tried, hit = b.autotune()
+        assert not hit and len(tried) == 8 and all(t[2] > 0 for t in tried)
+        assert cfg.queue_paths in [t[0] for t in tried]
+        cfg2, tried2, hit2 = b.autotune()
+        assert hit2 and not tried2 and (cfg2.queue_paths, cfg2.material_sort) == (cfg.queue_paths, cfg.material_sort)
+        # the calibration renders leave no trace in the images
+        st = b.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort)
+        s, w, h, spp, depth = jobs[0]
+        ref, _ = solo(s, w, h, spp, depth)
+        assert np.abs(b.image(0) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-6)
+        assert st.total_rays > 0
+    finally:
+        del os.environ["RM_TUNE_CACHE"]
