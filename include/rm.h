@@ -208,6 +208,7 @@ typedef struct rm_uniforms {
     uint32_t timing;          /* record per-stage GPU times (hipEvents) */
     uint32_t traversal_stats; /* count BVH nodes / triangles visited (slower) */
     uint32_t material_sort;   /* group each iteration's hits by material before shading */
+    uint32_t use_graph;       /* replay the bounce loop from a captured hipGraph (ignored with timing) */
 } rm_uniforms;
 
 typedef struct rm_render_stats {
@@ -330,6 +331,7 @@ typedef struct rm_batch_config {
     uint32_t material_sort;  /* group each iteration's hits by material */
     uint32_t rr_depth;       /* Russian roulette from this depth (0 = from the first bounce) */
     uint32_t timing;         /* per-stage GPU times */
+    uint32_t use_graph;      /* replay each bounce loop from a captured hipGraph (ignored with timing) */
 } rm_batch_config;
 
 typedef struct rm_batch_stats {

@@ -85,7 +85,8 @@ class SceneStats(C.Structure):
 class Uniforms(C.Structure):
     _fields_ = [("sampling_mode", C.c_uint32), ("accumulate", C.c_uint32), ("exposure", C.c_float),
                 ("tonemapping", C.c_uint32), ("rr_depth", C.c_uint32), ("accumulate_f64", C.c_uint32),
-                ("timing", C.c_uint32), ("traversal_stats", C.c_uint32), ("material_sort", C.c_uint32)]
+                ("timing", C.c_uint32), ("traversal_stats", C.c_uint32), ("material_sort", C.c_uint32),
+                ("use_graph", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -115,7 +116,7 @@ class JobDesc(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("rr_depth", C.c_uint32),
-                ("timing", C.c_uint32)]
+                ("timing", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -483,8 +484,8 @@ class Batch:
         self.handle = C.c_void_p()
         _check(LIB.rm_batch_create(device.handle, arr, len(jobs), C.byref(self.handle)))
 
-    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0):
-        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing)
+    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0, use_graph=0):
+        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing, use_graph)
         st = BatchStats()
         _check(LIB.rm_batch_render(self.handle, C.byref(cfg), C.byref(st)))
         return st
@@ -501,7 +502,7 @@ class Batch:
         return out
 
     def autotune(self, rr_depth=0):
-        cfg = BatchConfig(0, 0, rr_depth, 0)
+        cfg = BatchConfig(0, 0, rr_depth, 0, 0)
         entries = (BatchTuneEntry * 16)()
         count = C.c_uint32(0)
         hit = C.c_int(0)

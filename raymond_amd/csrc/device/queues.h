@@ -74,9 +74,16 @@ struct Counters {
     uint32_t sort_cursor[2][kSortBins];
     // radiance cache: lookups and training vertices recorded in this stream
     uint32_t cache_queries, train_records;
+    // first iteration of the chunk a captured bounce-loop graph is running
+    uint32_t iter_base, pad_iter;
     // traversal statistics (only written by the statistics variants)
     unsigned long long nodes_closest, tris_closest, nodes_any, tris_any;
 };
+
+// Iteration a kernel works on: `iter` itself when launched directly, or the
+// offset `iter` within a captured chunk whose first iteration the device
+// counter holds.
+RM_HD uint32_t iteration_of(const uint32_t* iter_dev, uint32_t iter) { return iter_dev ? *iter_dev + iter : iter; }
 
 // Render-time settings (bridge/Uniforms.hpp:29-47 plus this build's knobs).
 struct FrameUniforms {

@@ -300,7 +300,8 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
         for (uint32_t j = threadIdx.x; j < kJobsLds; j += kShadeBlock) job_ended[j] = 0u;
         __syncthreads();
     }
-    const uint32_t count = a.counters->it[a.iter].queue;
+    const uint32_t iter = iteration_of(a.iter_dev, a.iter);
+    const uint32_t count = a.counters->it[iter].queue;
     const uint32_t stride = gridDim.x * kShadeBlock;
     for (uint32_t base = blockIdx.x * kShadeBlock; base < count; base += stride) {
         const uint32_t k = base + threadIdx.x;
@@ -310,7 +311,7 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
             if (out.job < kJobsLds) atomicAdd(&job_ended[out.job], 1u);
             else atomicAdd(&a.batch.done[out.job], 1u);
         }
-        uint32_t* const ctr[4] = {&a.counters->it[a.iter].shadows, &a.counters->it[a.iter + 1].appended,
+        uint32_t* const ctr[4] = {&a.counters->it[iter].shadows, &a.counters->it[iter + 1].appended,
                                   &a.counters->cache_queries, &a.counters->train_records};
         const bool want[4] = {out.shadow, out.next, out.cache_kind == 1u, out.cache_kind == 2u};
         uint32_t at[4];

@@ -22,6 +22,7 @@ struct ShadeArgs {
     float* image;          // RGBA accumulation
     FrameUniforms u;
     uint32_t iter;             // iteration of the regenerating bounce loop
+    const uint32_t* iter_dev;  // captured bounce loops: iteration = *iter_dev + iter
 };
 
 struct EnvmapArgs {

@@ -204,6 +204,7 @@ void batch_render(rm_batch* b, const rm_batch_config* cfg, rm_batch_stats* stats
     r->u.rr_depth = c.rr_depth;
     r->u.material_sort = c.material_sort;
     r->u.timing = c.timing;
+    r->u.use_graph = c.use_graph;
     r->u.traversal_stats = 0;
     renderer_reset(r);
     const uint32_t n = (uint32_t)b->jobs.size();

@@ -295,62 +295,109 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     aa.counters = cnt;
     if (br) aa.batch = br->view;
 
+    // One iteration: regenerate, closest hit, material sort, shade, shadow
+    // rays.  Directly launched, `it` is the iteration; inside a captured chunk
+    // it is the offset from the device counter iter_base (chunks start at even
+    // iterations, so the queue parity is the offset's).
+    const uint32_t* iter_dev = nullptr;
+    auto enqueue = [&](uint32_t it) {
+        PathQueue& cur = r->q[it & 1];
+        PathQueue& next = r->q[(it + 1) & 1];
+        ga.iter = it;
+        ga.iter_dev = iter_dev;
+        ga.paths = cur;
+        ga.slots = slot_arr[it & 1];
+        RM_HIP(launch_regenerate(ga, cus * 4, st));
+        ca.iter = it;
+        ca.iter_dev = iter_dev;
+        ca.count_ptr = iter_dev ? nullptr : &cnt->it[it].queue;
+        ca.fetch = iter_dev ? nullptr : &cnt->it[it].fetch_closest;
+        ca.paths = cur;
+        hipEvent_t e = tm.begin();
+        RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
+        tm.end(e, tm.closest);
+        if (sorting) {
+            so.iter = it;
+            so.iter_dev = iter_dev;
+            hipEvent_t eo = tm.begin();
+            RM_HIP(launch_sort_keys(so, sort_blocks, st));
+            RM_HIP(launch_sort_scatter(so, sort_blocks, st));
+            tm.end(eo, tm.sort);
+        }
+        ShadeArgs sa;
+        sa.ctx = s->view;
+        sa.cur = cur;
+        sa.hits = r->hits;
+        sa.next = next;
+        sa.shadows = r->shadows;
+        sa.counters = cnt;
+        sa.order = sorting ? so.order : nullptr;
+        sa.cache = CacheView{};
+        sa.batch = br ? br->view : BatchView{};
+        if (cached) {
+            sa.cache = r->cv;
+            sa.cache.slots = slot_arr[it & 1];
+            sa.cache.slots_next = slot_arr[(it + 1) & 1];
+            sa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
+        }
+        sa.image = image;
+        sa.u = fu;
+        sa.iter = it;
+        sa.iter_dev = iter_dev;
+        void* args[] = {&sa};
+        hipEvent_t es = tm.begin();
+        RM_HIP(hipModuleLaunchKernel(shade_fn, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
+        tm.end(es, tm.shade);
+        if (cached) {
+            aa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
+            aa.slot_radiance = cache_train ? r->cv.slot_radiance : nullptr;
+        }
+        aa.iter = it;
+        aa.iter_dev = iter_dev;
+        aa.count_ptr = iter_dev ? nullptr : &cnt->it[it].shadows;
+        aa.fetch = iter_dev ? nullptr : &cnt->it[it].fetch_any;
+        hipEvent_t ea = tm.begin();
+        RM_HIP(launch_trace_any(aa, ablocks, tstats, st));
+        tm.end(ea, tm.any);
+    };
+
+    // Captured bounce loop: a chunk of kIterChunk iterations and the advance
+    // of iter_base, recorded once per stream and replayed until it drains.
+    // The graph holds kernels only (no memset nodes).
+    hipGraphExec_t chunk_exec = nullptr;
+    if (r->u.use_graph && !tm.on) {
+        iter_dev = &cnt->iter_base;
+        hipGraph_t g = nullptr;
+        RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            for (uint32_t c = 0; c < kIterChunk; c++) enqueue(c);
+            RM_HIP(launch_advance_iter(cnt, kIterChunk, st));
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        RM_HIP(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&chunk_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        RM_HIP(e);
+    }
+    struct ExecGuard {
+        hipGraphExec_t& e;
+        ~ExecGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } exec_guard{chunk_exec};
+
     uint32_t iter = 0;
     std::vector<IterCounters> host_it;
     for (;;) {
         RM_REQUIRE(iter + kIterChunk + 1 < (uint32_t)kMaxIters, "render stream exceeded the iteration limit");
-        for (uint32_t c = 0; c < kIterChunk; c++, iter++) {
-            PathQueue& cur = r->q[iter & 1];
-            PathQueue& next = r->q[(iter + 1) & 1];
-            ga.iter = iter;
-            ga.paths = cur;
-            ga.slots = slot_arr[iter & 1];
-            RM_HIP(launch_regenerate(ga, cus * 4, st));
-            ca.count_ptr = &cnt->it[iter].queue;
-            ca.fetch = &cnt->it[iter].fetch_closest;
-            ca.paths = cur;
-            hipEvent_t e = tm.begin();
-            RM_HIP(launch_trace_closest(ca, tblocks, tstats, st));
-            tm.end(e, tm.closest);
-            if (sorting) {
-                so.iter = iter;
-                hipEvent_t eo = tm.begin();
-                RM_HIP(launch_sort_keys(so, sort_blocks, st));
-                RM_HIP(launch_sort_scatter(so, sort_blocks, st));
-                tm.end(eo, tm.sort);
-            }
-            ShadeArgs sa;
-            sa.ctx = s->view;
-            sa.cur = cur;
-            sa.hits = r->hits;
-            sa.next = next;
-            sa.shadows = r->shadows;
-            sa.counters = cnt;
-            sa.order = sorting ? so.order : nullptr;
-            sa.cache = CacheView{};
-            sa.batch = br ? br->view : BatchView{};
-            if (cached) {
-                sa.cache = r->cv;
-                sa.cache.slots = slot_arr[iter & 1];
-                sa.cache.slots_next = slot_arr[(iter + 1) & 1];
-                sa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
-            }
-            sa.image = image;
-            sa.u = fu;
-            sa.iter = iter;
-            void* args[] = {&sa};
-            hipEvent_t es = tm.begin();
-            RM_HIP(hipModuleLaunchKernel(shade_fn, sblocks, 1, 1, 256, 1, 1, 0, st, args, nullptr));
-            tm.end(es, tm.shade);
-            if (cached) {
-                aa.shadows.slot = cache_train ? r->shadow_slot_buf.as<uint32_t>() : nullptr;
-                aa.slot_radiance = cache_train ? r->cv.slot_radiance : nullptr;
-            }
-            aa.count_ptr = &cnt->it[iter].shadows;
-            aa.fetch = &cnt->it[iter].fetch_any;
-            hipEvent_t ea = tm.begin();
-            RM_HIP(launch_trace_any(aa, ablocks, tstats, st));
-            tm.end(ea, tm.any);
+        if (chunk_exec) {
+            RM_HIP(hipGraphLaunch(chunk_exec, st));
+            iter += kIterChunk;
+        } else {
+            for (uint32_t c = 0; c < kIterChunk; c++, iter++) enqueue(iter);
         }
         IterCounters probe{};
         RM_HIP(hipMemcpyAsync(&probe, &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));

@@ -79,6 +79,16 @@ hipError_t launch_clear_counters(Counters* c, hipStream_t s) {
     hipLaunchKernelGGL(clear_counters_kernel, dim3(64), dim3(256), 0, s, c);
     return hipGetLastError();
 }
+// Moves a captured bounce-loop graph on to its next chunk of iterations.
+__global__ void advance_iter_kernel(Counters* c, uint32_t n) {
+    if (threadIdx.x == 0) c->iter_base += n;
+}
+
+hipError_t launch_advance_iter(Counters* c, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(advance_iter_kernel, dim3(1), dim3(64), 0, s, c, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_envmap_reduce(float* level, int res, hipStream_t s) {
     const uint32_t n = (uint32_t)(res * res);
     hipLaunchKernelGGL(envmap_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, level, res);

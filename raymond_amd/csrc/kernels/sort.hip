@@ -22,7 +22,8 @@ __global__ __launch_bounds__(kSortBlock) void sort_keys_kernel(SortArgs a) {
     const uint32_t t = threadIdx.x;
     hist[t] = 0u;
     __syncthreads();
-    const uint32_t count = a.counters->it[a.iter].queue;
+    const uint32_t iter = iteration_of(a.iter_dev, a.iter);
+    const uint32_t count = a.counters->it[iter].queue;
     const uint32_t stride = gridDim.x * kSortBlock;
     for (uint32_t i = blockIdx.x * kSortBlock + t; i < count; i += stride) {
         uint32_t key = kSortBins - 1;  // miss: the world shader
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(kSortBlock) void sort_keys_kernel(SortArgs a) {
         atomicAdd(&hist[key], 1u);
     }
     __syncthreads();
-    if (hist[t]) atomicAdd(&a.counters->sort_bins[a.iter & 1u][t], hist[t]);
+    if (hist[t]) atomicAdd(&a.counters->sort_bins[iter & 1u][t], hist[t]);
 }
 
 __global__ __launch_bounds__(kSortBlock) void sort_scatter_kernel(SortArgs a) {
@@ -43,7 +44,8 @@ __global__ __launch_bounds__(kSortBlock) void sort_scatter_kernel(SortArgs a) {
     __shared__ uint32_t start[kSortBins];
     __shared__ uint32_t cnt[kSortBins];
     const uint32_t t = threadIdx.x;
-    const uint32_t par = a.iter & 1u;
+    const uint32_t iter = iteration_of(a.iter_dev, a.iter);
+    const uint32_t par = iter & 1u;
 
     // exclusive prefix over the bins (Hillis-Steele in LDS)
     const uint32_t own = a.counters->sort_bins[par][t];
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(kSortBlock) void sort_scatter_kernel(SortArgs a) {
     start[t] = scan[t] - own;
 
     // this workgroup's count per bin, then one claim per non-empty bin
-    const uint32_t count = a.counters->it[a.iter].queue;
+    const uint32_t count = a.counters->it[iter].queue;
     const uint32_t stride = gridDim.x * kSortBlock;
     for (uint32_t i = blockIdx.x * kSortBlock + t; i < count; i += stride) atomicAdd(&cnt[a.keys[i]], 1u);
     __syncthreads();

@@ -17,6 +17,7 @@ struct SortArgs {
     uint32_t* order;            // queue entries grouped by key
     Counters* counters;
     uint32_t iter;
+    const uint32_t* iter_dev;   // captured bounce loops (iteration_of)
 };
 
 // keys(iter) then scatter(iter); both read the queue length from the

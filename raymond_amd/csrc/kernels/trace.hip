@@ -191,12 +191,19 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
-    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    const uint32_t* count_ptr = a.count_ptr;
+    uint32_t* fetch = a.fetch;
+    if (a.iter_dev) {
+        IterCounters& ic = a.counters->it[iteration_of(a.iter_dev, a.iter)];
+        count_ptr = &ic.queue;
+        fetch = &ic.fetch_closest;
+    }
+    const uint32_t count = count_ptr ? *count_ptr : a.count;
     unsigned long long sn = 0, st = 0;
     BatchScheduler sched(count);
     if (blockIdx.x >= sched.nb) return;
     while (true) {
-        const uint32_t batch = sched.next(a.fetch);
+        const uint32_t batch = sched.next(fetch);
         if (batch >= sched.nb) break;
         const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
         if (i >= count) continue;
@@ -243,7 +250,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
 // pixel with block-linear index g mod (W H), keyed exactly as the
 // reference's raygen keys (frame seed, rayIndex) (generate.metal:88-121).
 __global__ void regenerate_kernel(RegenArgs a) {
-    const IterCounters& cur = a.counters->it[a.iter];
+    const uint32_t iter = iteration_of(a.iter_dev, a.iter);
+    const IterCounters& cur = a.counters->it[iter];
     const uint32_t appended = cur.appended;
     const uint32_t generated = cur.generated;
     const uint32_t room = a.capacity - appended;
@@ -251,8 +259,8 @@ __global__ void regenerate_kernel(RegenArgs a) {
     const uint32_t k = room < left ? room : left;
     const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t0 == 0) {
-        a.counters->it[a.iter].queue = appended + k;
-        a.counters->it[a.iter + 1].generated = generated + k;
+        a.counters->it[iter].queue = appended + k;
+        a.counters->it[iter + 1].generated = generated + k;
     }
     const uint32_t wh = a.width * a.height;
     for (uint32_t t = t0; t < k; t += gridDim.x * blockDim.x) {
@@ -309,12 +317,19 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
-    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    const uint32_t* count_ptr = a.count_ptr;
+    uint32_t* fetch = a.fetch;
+    if (a.iter_dev) {
+        IterCounters& ic = a.counters->it[iteration_of(a.iter_dev, a.iter)];
+        count_ptr = &ic.shadows;
+        fetch = &ic.fetch_any;
+    }
+    const uint32_t count = count_ptr ? *count_ptr : a.count;
     unsigned long long sn = 0, st = 0;
     BatchScheduler sched(count);
     if (blockIdx.x >= sched.nb) return;
     while (true) {
-        const uint32_t batch = sched.next(a.fetch);
+        const uint32_t batch = sched.next(fetch);
         if (batch >= sched.nb) break;
         const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
         if (i >= count) continue;

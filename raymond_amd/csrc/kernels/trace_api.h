@@ -24,11 +24,15 @@ struct ClosestArgs {
     Counters* counters;
     Camera camera;              // clip range of depth-0 paths
     BatchView batch;            // batched jobs (jobs null: one scene, root 0)
+    // captured bounce loops: count and fetch come from counters->it[iteration_of(iter_dev, iter)]
+    const uint32_t* iter_dev;
+    uint32_t iter;
 };
 
 struct RegenArgs {
     Counters* counters;
     uint32_t iter;
+    const uint32_t* iter_dev;   // captured bounce loops (iteration_of)
     uint32_t capacity;          // queue capacity
     uint32_t total;             // paths in the stream (W H samples)
     PathQueue paths;            // queue of iteration `iter`
@@ -58,6 +62,8 @@ struct AnyArgs {
     uint2* spill;
     Counters* counters;
     BatchView batch;
+    const uint32_t* iter_dev;   // captured bounce loops, as in ClosestArgs
+    uint32_t iter;
 };
 
 hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream);

@@ -36,10 +36,10 @@ def solo(scene, w, h, spp, depth):
     return r.image(), st.total_rays
 
 
-@pytest.mark.parametrize("sort", [0, 1])
-def test_batch_matches_solo_renders(device, jobs, sort):
+@pytest.mark.parametrize("sort,graph", [(0, 0), (1, 0), (1, 1)])
+def test_batch_matches_solo_renders(device, jobs, sort, graph):
     b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
-    st = b.render(queue_paths=4096, material_sort=sort)
+    st = b.render(queue_paths=4096, material_sort=sort, use_graph=graph)
     total = 0
     for j, (s, w, h, spp, depth) in enumerate(jobs):
         ref, rays = solo(s, w, h, spp, depth)
@@ -56,7 +56,7 @@ def test_batch_splits_into_bounce_loops(device, jobs):
     """A queue much smaller than the batch forces several bounce loops; the
     result is the same."""
     b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
-    st = b.render(queue_paths=64)
+    st = b.render(queue_paths=16)
     assert st.loops > 1
     s, w, h, spp, depth = jobs[1]
     ref, _ = solo(s, w, h, spp, depth)

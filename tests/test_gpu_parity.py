@@ -159,13 +159,15 @@ def test_render_spheres_per_sample(small_spheres):
     assert np.abs(r.image() - ref).max() <= 1e-4 * scale
 
 
-@pytest.mark.parametrize("spl,samples", [(1, 3), (2, 5)])
-def test_regenerating_stream_matches_frames(cornell, spl, samples):
+@pytest.mark.parametrize("spl,samples,graph", [(1, 3, 0), (2, 5, 0), (2, 5, 1)])
+def test_regenerating_stream_matches_frames(cornell, spl, samples, graph):
     """A stream of `samples` spp through a queue of spl frames (paths are
-    regenerated as others terminate) equals the sum of per-frame oracle renders."""
+    regenerated as others terminate) equals the sum of per-frame oracle
+    renders, launched directly or replayed from a captured graph."""
     desc, scene, orc = cornell
     W, H = 64, 64
     r = api.Renderer(scene, W, H, max_depth=8, samples_per_launch=spl)
+    r.set_uniforms(use_graph=graph)
     st = r.execute(samples)
     ref, res = orc.render(W, H, 8, frames=samples, frame_begin=0)
     assert st.total_rays == res.total_rays
@@ -220,3 +222,23 @@ def test_material_sort_matches_unsorted(graphs):
     assert out[0][1] == out[1][1]
     scale = max(np.abs(out[0][2]).max(), 1e-6)
     assert np.abs(out[0][2] - out[1][2]).max() <= 1e-5 * scale
+
+
+def test_graph_replay_matches_direct_launches(cornell):
+    """Several streams, each captured once and replayed chunk by chunk with
+    host work between the replays, match direct launches iteration by
+    iteration (ray counts) and in the image."""
+    desc, scene, orc = cornell
+    W, H = 64, 64
+    out = []
+    for graph in (0, 1):
+        r = api.Renderer(scene, W, H, max_depth=8, samples_per_launch=1)
+        r.set_uniforms(use_graph=graph, rr_depth=3)
+        counts = []
+        for _ in range(3):
+            st = r.execute(4)
+            counts.append([st.rays[d] for d in range(33)] + [st.shadow_rays[d] for d in range(32)])
+        out.append((counts, r.image()))
+    assert out[0][0] == out[1][0]
+    scale = max(np.abs(out[0][1]).max(), 1e-6)
+    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-5 * scale
