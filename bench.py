@@ -26,6 +26,17 @@ NODE_BYTES, TRI_BYTES = 80, 48
 RAY_READ, HIT_WRITE = 28, 20  # origin + direction + meta word; t, prim, inst, 2 barycentrics
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (copy_probe_kernel reads and
+    writes mib MiB), the reference a hardware byte counter is scaled by."""
+    import raymond_amd.api as api
+    dev = api.Device(0)
+    ms = dev.copy_probe(mib << 20)
+    print("copy probe: %d MiB read and written in %.3f ms (%.0f GB/s)" % (mib, ms, 2 * (mib << 20) / ms / 1e6),
+          file=sys.stderr, flush=True)
+    dev.close()
+
+
 def workload_name(graphs, W, H, D, sort):
     if graphs:
         return ("shader graphs (config 3): Cornell-box geometry, 32 random 20-60 node Cycles graphs (seed 3) "
@@ -126,14 +137,17 @@ def bench_cache(args):
         r = api.Renderer(scene, W, H, max_depth=D)
         r.set_uniforms(rr_depth=3)
         cache = None
+        def new_cache():
+            return api.RadianceCache(dev, seed=1, batch_size=args.cache_batch, steps_per_launch=args.cache_steps)
+
         if cached:
-            cache = api.RadianceCache(dev, seed=1)
+            cache = new_cache()
             r.set_cache(cache, mode=1)
         r.execute(1)  # warm-up (first launches, cache training starts)
         r.reset()
         if cache is not None:
             cache.close()
-            cache = api.RadianceCache(dev, seed=1)
+            cache = new_cache()
             r.set_cache(cache, mode=1)
         points, done, spent, rays = [], 0, 0.0, 0
         k = 1
@@ -188,7 +202,8 @@ def bench_cache(args):
         "data": "synthetic",
         "config": {"workload": "radiance cache (config 5): Cornell box %dx%d, depth %d, roulette from bounce 3, "
                                "hash grid 16x2 FP16 x 2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 "
-                               "trains; reference: plain tracer, %d spp, FP64 accumulation" % (W, H, D, args.ref_spp)},
+                               "trains, %d training steps of %d samples per spp; reference: plain tracer, %d spp, "
+                               "FP64 accumulation" % (W, H, D, args.cache_steps, args.cache_batch, args.ref_spp)},
         "plain": plain,
         "cached": cached,
         "plain_rmse_at_steps": target,
@@ -213,11 +228,15 @@ def main():
                          "batch: 64 uneven Cornell-box jobs in one bounce loop")
     ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
+    ap.add_argument("--cache-batch", type=int, default=16384, help="cache workload: samples per training step")
+    ap.add_argument("--cache-steps", type=int, default=4, help="cache workload: training steps per sample")
     ap.add_argument("--material-sort", type=int, default=-1,
                     help="group hits by material before shading (-1 = on for graphs, off for spheres)")
     ap.add_argument("--inline-materials", type=int, default=-1,
                     help="1: material graphs inlined into the shading kernel, 0: out-of-line calls, -1: by size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy a buffer of this many MiB on the device (calibrates HBM counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-pixels", type=int, default=0, help="oracle sample size (0 = full frames)")
     ap.add_argument("--cpu-frames", type=int, default=16, help="oracle sample: frames (spp) of the full image")
@@ -235,6 +254,8 @@ def main():
         return bench_cache(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     W, H = args.width, args.height
     graphs = args.workload == "graphs"
     D = args.depth or (8 if graphs else 12)
@@ -284,6 +305,16 @@ def main():
                                                       nodes_per_ray, tris_per_ray),
           file=sys.stderr, flush=True)
 
+    # HBM traffic of trace_closest: bytes per ray from the FETCH_SIZE /
+    # WRITE_SIZE counter runs recorded for this workload, times this run's
+    # closest rays per launch
+    traffic = None
+    traffic_file = os.path.join(HERE, "profiles", "trace_closest_c2_traffic.json")
+    if not graphs and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        traffic = tr["traffic_bytes_per_ray"] * closest_total / max(totals["nclosest"], 1)
+
     cpu = None
     if not args.no_cpu:
         from oracle import binding
@@ -325,13 +356,17 @@ def main():
             "iterations": int(totals["iters"]), "material_sort": bool(sort),
         },
         "samples_per_s": round(samples_per_s, 1),
+        "rays": {"closest": int(totals["closest"]), "shadow": int(totals["shadow"]),
+                 "closest_launches": int(totals["nclosest"])},
         "stage_ms_per_step": {"trace_closest": round(totals["ms_closest"] / args.steps, 3),
                               "material_sort": round(totals["ms_sort"] / args.steps, 3),
                               "shade": round(totals["ms_shade"] / args.steps, 3),
                               "trace_any_shadow": round(totals["ms_any"] / args.steps, 3)},
         "roofline": {"kernel": "trace_closest", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_ray": round(bytes_per_ray, 1),
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_source": None if traffic is None else "profiles/trace_closest_c2_traffic.json",
+                     "bytes_per_ray": round(bytes_per_ray, 1),
                      "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2)},
         "cpu_baseline": cpu,
     }

@@ -252,6 +252,11 @@ int rm_read_image_f64(rm_renderer* r, double* rgba, size_t count);
 int rm_resolve(rm_renderer* r, float* rgba, size_t count);
 uint32_t rm_renderer_frame_index(rm_renderer* r);
 
+/* Copies `bytes` (a multiple of 16) from one device buffer to another with a
+ * streaming kernel (copy_probe_kernel) and returns its time: the known-size
+ * reference that scales hardware byte counters.  No reference counterpart. */
+int rm_device_copy_probe(rm_device* dev, size_t bytes, double* ms);
+
 /* ------------------------------------------------------------------------
  * Learned radiance cache.  No reference counterpart: the reference traces
  * every path to termination.  A multiresolution hash-grid encoding (16 levels

@@ -165,6 +165,7 @@ EXPORTS = {
     "rm_read_image_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "rm_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "rm_renderer_frame_index": (C.c_uint32, [C.c_void_p]),
+    "rm_device_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_double)]),
     "rm_cache_create": (C.c_int, [C.c_void_p, C.POINTER(CacheDesc), C.POINTER(C.c_void_p)]),
     "rm_cache_destroy": (None, [C.c_void_p]),
     "rm_cache_get_stats": (C.c_int, [C.c_void_p, C.POINTER(CacheStats)]),
@@ -262,6 +263,12 @@ class Device:
     def __init__(self, ordinal=0):
         self.handle = C.c_void_p()
         _check(LIB.rm_device_create(ordinal, C.byref(self.handle)))
+
+    def copy_probe(self, nbytes):
+        """Streaming device copy of nbytes (reads and writes nbytes); returns ms."""
+        ms = C.c_double(0.0)
+        _check(LIB.rm_device_copy_probe(self.handle, nbytes, C.byref(ms)))
+        return ms.value
 
     def info(self):
         i = DeviceInfo()

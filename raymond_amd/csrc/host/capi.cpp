@@ -3,6 +3,7 @@
 #include <cstring>
 #include <string>
 
+#include "../kernels/misc_api.h"
 #include "../kernels/trace_api.h"
 #include "core.h"
 #include "batch_api.h"
@@ -284,6 +285,34 @@ int rm_resolve(rm_renderer* r, float* rgba, size_t count) {
 }
 
 uint32_t rm_renderer_frame_index(rm_renderer* r) { return r ? r->frame_index : 0; }
+
+int rm_device_copy_probe(rm_device* dev, size_t bytes, double* ms) {
+    RM_NULL_CHECK(dev);
+    if (bytes == 0 || bytes % 16) {
+        rm::set_error("copy size must be a positive multiple of 16 bytes");
+        return RM_ERROR_INVALID;
+    }
+    RM_GUARD_BEGIN
+    hipStream_t st = dev->stream;
+    rm::DevBuf src, dst;
+    src.alloc(bytes);
+    dst.alloc(bytes);
+    RM_HIP(hipMemsetAsync(src.ptr, 0x3c, bytes, st));
+    hipEvent_t e0, e1;
+    RM_HIP(hipEventCreate(&e0));
+    RM_HIP(hipEventCreate(&e1));
+    RM_HIP(hipEventRecord(e0, st));
+    const hipError_t err = rm::launch_copy_probe(src.ptr, dst.ptr, bytes, st);
+    RM_HIP(hipEventRecord(e1, st));
+    RM_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RM_HIP(err);
+    if (ms) *ms = t;
+    RM_GUARD_END
+}
 
 int rm_cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache** out) {
     RM_NULL_CHECK(dev);

@@ -79,6 +79,19 @@ hipError_t launch_clear_counters(Counters* c, hipStream_t s) {
     hipLaunchKernelGGL(clear_counters_kernel, dim3(64), dim3(256), 0, s, c);
     return hipGetLastError();
 }
+// Streaming copy of n 16-byte words: the reference point for hardware byte
+// counters (it reads and writes exactly 16 n bytes).
+__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, s, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(dst), bytes / 16);
+    return hipGetLastError();
+}
+
 // Moves a captured bounce-loop graph on to its next chunk of iterations.
 __global__ void advance_iter_kernel(Counters* c, uint32_t n) {
     if (threadIdx.x == 0) c->iter_base += n;

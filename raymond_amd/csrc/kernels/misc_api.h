@@ -7,6 +7,7 @@ namespace rm {
 
 hipError_t launch_clear_counters(Counters* c, hipStream_t s);
 hipError_t launch_advance_iter(Counters* c, uint32_t n, hipStream_t s);
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_envmap_reduce(float* level, int res, hipStream_t s);
 hipError_t launch_envmap_normalize(const float* total, float* pdfs, uint32_t n, hipStream_t s);
 hipError_t launch_resolve(const float* img, float* out, uint32_t n, float exposure, float divisor, int tonemap,

@@ -210,7 +210,8 @@ __device__ inline void rc_image_add(float* image, uint32_t pixel, float r, float
     atomicAdd(px + 3, 1.0f);
 }
 
-__global__ __launch_bounds__(64 * kInferWaves) void rc_infer_kernel(RcInferArgs a) {
+// Three waves per SIMD is the most the register budget allows without spills.
+__global__ __launch_bounds__(64 * kInferWaves, 3) void rc_infer_kernel(RcInferArgs a) {
     __shared__ RcLdsFwd W;
     load_fwd_weights(W, a.p.mlp_h);
     __syncthreads();
