@@ -235,6 +235,7 @@ def main():
     ap.add_argument("--inline-materials", type=int, default=-1,
                     help="1: material graphs inlined into the shading kernel, 0: out-of-line calls, -1: by size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--graph", type=int, default=0, help="replay the bounce loop from a captured hipGraph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy a buffer of this many MiB on the device (calibrates HBM counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -275,7 +276,7 @@ def main():
 
     spl = args.spp_per_launch
     r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
-    r.set_uniforms(timing=1, rr_depth=0, material_sort=sort)
+    r.set_uniforms(timing=0 if args.graph else 1, rr_depth=0, material_sort=sort, use_graph=args.graph)
     if args.warmup:
         r.execute(args.warmup)
     # the timed steps run as one regenerating stream of `steps` samples per pixel
@@ -313,7 +314,8 @@ def main():
     if not graphs and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
-        traffic = tr["traffic_bytes_per_ray"] * closest_total / max(totals["nclosest"], 1)
+        # one closest-hit launch per bounce-loop iteration
+        traffic = tr["traffic_bytes_per_ray"] * closest_total / max(totals["iters"], 1)
 
     cpu = None
     if not args.no_cpu:
@@ -357,7 +359,7 @@ def main():
         },
         "samples_per_s": round(samples_per_s, 1),
         "rays": {"closest": int(totals["closest"]), "shadow": int(totals["shadow"]),
-                 "closest_launches": int(totals["nclosest"])},
+                 "closest_launches": int(totals["iters"])},
         "stage_ms_per_step": {"trace_closest": round(totals["ms_closest"] / args.steps, 3),
                               "material_sort": round(totals["ms_sort"] / args.steps, 3),
                               "shade": round(totals["ms_shade"] / args.steps, 3),

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libraymond_amd.so")
+# RM_LIBRARY selects another build of the library (kernel variants under test)
+LIB_PATH = os.environ.get("RM_LIBRARY") or os.path.join(HERE, "libraymond_amd.so")
 
 RAY_CAMERA, RAY_REFLECTION, RAY_TRANSMISSION, RAY_SHADOW = 1, 2, 4, 8
 RAY_VOLUME, RAY_DIFFUSE, RAY_GLOSSY, RAY_SINGULAR, RAY_ALL = 16, 32, 64, 128, 255

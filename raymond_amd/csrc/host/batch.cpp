@@ -139,7 +139,7 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
     m->instances = instances;
     const int bpc = std::max(dev->trace_blocks_per_cu, dev->any_blocks_per_cu);
     m->spill_threads = (size_t)dev->props.multiProcessorCount * (size_t)bpc * kTraceBlock;
-    m->d_spill.alloc(m->spill_threads * (size_t)(kMaxTraceStack - 16) * sizeof(uint2));
+    m->d_spill.alloc(m->spill_threads * (size_t)(kMaxTraceStack - kLdsStack) * sizeof(uint2));
     m->stats.triangles = tris.size();
     m->stats.bvh_nodes = nodes.size();
 

@@ -251,7 +251,7 @@ void scene_build_accel(rm_scene* s, int threads) {
     // spill rows of the traversal stack for the largest persistent grid
     const int bpc = std::max(s->dev->trace_blocks_per_cu, s->dev->any_blocks_per_cu);
     s->spill_threads = (size_t)s->dev->props.multiProcessorCount * (size_t)bpc * kTraceBlock;
-    s->d_spill.alloc(s->spill_threads * (size_t)(kMaxTraceStack - 16) * sizeof(uint2));
+    s->d_spill.alloc(s->spill_threads * (size_t)(kMaxTraceStack - kLdsStack) * sizeof(uint2));
     RM_HIP(hipStreamSynchronize(st));
     s->accel_built = true;
     s->stats.triangles = tris.size();

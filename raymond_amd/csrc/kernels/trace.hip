@@ -27,8 +27,6 @@
 
 namespace rm {
 
-constexpr int kLdsStack = 16;
-
 __device__ inline uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
     const uint32_t w = c < 4 ? lo : hi;
     return (w >> (8 * (c & 3))) & 0xffu;

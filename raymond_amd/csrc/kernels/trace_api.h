@@ -10,6 +10,10 @@ namespace rm {
 
 constexpr int kTraceBlock = 64;      // one wave64 per workgroup
 constexpr int kMaxTraceStack = 64;   // LDS rows + spill rows; BVH depth is bounded below this
+// Stack rows per lane in LDS.  The stack holds at most one entry per tree
+// level, so 12 rows cover BVHs up to depth 12 without spilling; 6 KB of LDS
+// per wave leaves the VGPRs, not LDS, limiting occupancy.
+constexpr int kLdsStack = 12;
 struct ClosestArgs {
     const Bvh8Node* nodes;
     const BvhTri* tris;
@@ -20,7 +24,7 @@ struct ClosestArgs {
     HitQueue hits;
     const float* tmin_arr;      // optional per-ray range (raw batches); else eps / inf
     const float* tmax_arr;
-    uint2* spill;               // (kMaxTraceStack - 16) x grid threads
+    uint2* spill;               // (kMaxTraceStack - kLdsStack) x grid threads
     Counters* counters;
     Camera camera;              // clip range of depth-0 paths
     BatchView batch;            // batched jobs (jobs null: one scene, root 0)
