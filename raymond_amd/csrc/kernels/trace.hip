@@ -45,30 +45,45 @@ struct StackRef {
     }
 };
 
-struct TraceResult {
-    float t;
-    uint32_t inst, prim;
-    float b0, b1;
-    uint32_t nodes, tris;
-};
+// Traversal state of one lane's ray.  It persists across steps, so a lane
+// whose ray has finished takes a new one while the rest of the wave goes on.
+struct RayState {
+    vec3 o, inv;
+    RayShear sh;
+    int oct;
+    float tmin, tmax, best_t;
+    uint32_t best_inst, best_prim;
+    float bb0, bb1;
+    bool found;
+    uint2 cur;  // node group: child base, (remaining child hits << 24) | internal mask
+    int sp;
+    uint32_t n_nodes, n_tris;
 
-// Returns true on a hit. ANY: stop at the first hit in (tmin, tmax).
-template <bool ANY, bool STATS>
-__device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, uint32_t root, vec3 o,
-                         vec3 d, float tmin, float tmax, const StackRef& stk, TraceResult& res) {
-    const vec3 inv = safe_inv_dir(d);
-    const RayShear sh = make_shear(d);
-    const int oct = dir_octant(d);
-    // a one-child pseudo node whose only child is the root
-    uint2 cur = make_uint2(root, (1u << (24 + oct)) | 1u);
-    int sp = 0;
-    float best_t = tmax;
-    uint32_t best_inst = 0xffffffffu, best_prim = 0xffffffffu;
-    float bb0 = 0.0f, bb1 = 0.0f;
-    bool found = false;
-    uint32_t n_nodes = 0, n_tris = 0;
+    __device__ void init(uint32_t root, vec3 org, vec3 d, float t0, float t1) {
+        o = org;
+        inv = safe_inv_dir(d);
+        sh = make_shear(d);
+        oct = dir_octant(d);
+        // a one-child pseudo node whose only child is the root
+        cur = make_uint2(root, (1u << (24 + oct)) | 1u);
+        sp = 0;
+        tmin = t0;
+        tmax = t1;
+        best_t = t1;
+        best_inst = 0xffffffffu;
+        best_prim = 0xffffffffu;
+        bb0 = 0.0f;
+        bb1 = 0.0f;
+        found = false;
+        n_nodes = 0;
+        n_tris = 0;
+    }
 
-    while (true) {
+    // One node and the triangles of its leaf children, or one stack pop.
+    // Returns true when the traversal has ended (ANY: also at the first hit
+    // in (tmin, tmax)).
+    template <bool ANY, bool STATS>
+    __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, const StackRef& stk) {
         uint2 tri = make_uint2(0u, 0u);
         if (cur.y & 0xff000000u) {
             const uint32_t hits = cur.y >> 24;
@@ -108,10 +123,10 @@ __device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __res
             cur = make_uint2(q1.x, (ihits << 24) | imask);
             tri = make_uint2(q1.y, thits);
         } else {
-            if (sp == 0) break;
+            if (sp == 0) return true;
             sp--;
             cur = stk.load(sp);
-            continue;
+            return false;
         }
         while (tri.y) {
             const int b = __builtin_ctz(tri.y);
@@ -131,24 +146,18 @@ __device__ bool traverse(const Bvh8Node* __restrict__ nodes, const BvhTri* __res
                 bb0 = b0;
                 bb1 = b1;
                 found = true;
-                if (ANY) {
-                    res.nodes = n_nodes;
-                    res.tris = n_tris;
-                    res.t = t;
-                    return true;
-                }
+                if (ANY) return true;
             }
         }
+        return false;
     }
-    res.t = found ? best_t : -1.0f;
-    res.inst = best_inst;
-    res.prim = best_prim;
-    res.b0 = bb0;
-    res.b1 = bb1;
-    res.nodes = n_nodes;
-    res.tris = n_tris;
-    return found;
-}
+};
+
+// Lanes are refilled once at least this many of a wave's 64 are idle.
+#ifndef RM_REFILL_LANES
+#define RM_REFILL_LANES 8
+#endif
+constexpr uint32_t kRefillLanes = RM_REFILL_LANES;
 
 // Batch scheduler of the persistent waves.  The first 3/4 of the 64-ray
 // batches are assigned statically (wave w takes w, w + nwaves, ...); the rest
@@ -180,6 +189,56 @@ struct BatchScheduler {
     }
 };
 
+// Rays of a wave: batches of 64 consecutive queue entries come from the
+// scheduler, and idle lanes take the batch's next rays in lane order.  A
+// refill happens once kRefillLanes lanes are idle (or all are), so the wave
+// pays the ray set-up for several lanes at a time.  Wave-uniform control.
+struct RayPool {
+    BatchScheduler& sched;
+    uint32_t* fetch;
+    uint32_t count;
+    uint32_t batch, taken;
+    bool exhausted;
+    uint32_t ray;  // the ray a lane was given by the last refill
+
+    __device__ RayPool(BatchScheduler& s, uint32_t* f, uint32_t n) : sched(s), fetch(f), count(n), taken(0) {
+        batch = sched.next(fetch);
+        exhausted = batch >= sched.nb;
+        ray = 0;
+    }
+    // Returns true for the lanes that received a new ray.
+    __device__ bool refill(bool active) {
+        if (exhausted) return false;
+        unsigned long long idle = __ballot(!active);
+        if ((uint32_t)__popcll(idle) < kRefillLanes && ~idle != 0ull) return false;
+        const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+        bool got = false;
+        while (idle != 0ull && !exhausted) {
+            const uint32_t first = batch * (uint32_t)kTraceBlock + taken;
+            const uint32_t room = (uint32_t)kTraceBlock - taken;
+            const uint32_t left = count > first ? count - first : 0u;
+            const uint32_t avail = room < left ? room : left;
+            if (avail == 0u) {
+                batch = sched.next(fetch);
+                taken = 0;
+                exhausted = batch >= sched.nb;
+                continue;
+            }
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            const uint32_t give = n_idle < avail ? n_idle : avail;
+            const bool is_idle = (idle >> __lane_id()) & 1ull;
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (is_idle && rank < give) {
+                ray = first + rank;
+                got = true;
+            }
+            taken += give;
+            idle = __ballot(!active && !got);
+        }
+        return got;
+    }
+};
+
 // Closest hit over a path queue.  Camera paths (depth 0) use the camera's
 // clip range, continuation rays (eps, inf) (intersection.metal:201-202);
 // raw batches bring their own ranges.
@@ -200,41 +259,52 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
     unsigned long long sn = 0, st = 0;
     BatchScheduler sched(count);
     if (blockIdx.x >= sched.nb) return;
+    const PathQueue& q = a.paths;
+    RayState s;
+    bool active = false;
+    uint32_t i = 0;
+    RayPool pool(sched, fetch, count);
     while (true) {
-        const uint32_t batch = sched.next(fetch);
-        if (batch >= sched.nb) break;
-        const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
-        if (i >= count) continue;
-        const PathQueue& q = a.paths;
-        const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
-        const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
-        uint32_t root = 0;
-        float near_clip = a.camera.near_clip, far_clip = a.camera.far_clip;
-        if (a.batch.jobs) {
-            const JobInfo& job = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])];
-            root = job.root;
-            near_clip = job.camera.near_clip;
-            far_clip = job.camera.far_clip;
+        // idle lanes take the next rays of the wave's batch
+        if (pool.refill(active)) {
+            i = pool.ray;
+            const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
+            const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
+            uint32_t root = 0;
+            float near_clip = a.camera.near_clip, far_clip = a.camera.far_clip;
+            if (a.batch.jobs) {
+                const JobInfo& job = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])];
+                root = job.root;
+                near_clip = job.camera.near_clip;
+                far_clip = job.camera.far_clip;
+            }
+            float tmin, tmax;
+            if (a.tmin_arr) {
+                tmin = a.tmin_arr[i];
+                tmax = a.tmax_arr[i];
+            } else if (meta_depth(q.meta[i]) == 0) {
+                tmin = near_clip;
+                tmax = far_clip;
+            } else {
+                tmin = kEps;
+                tmax = kInf;
+            }
+            s.init(root, o, d, tmin, tmax);
+            active = true;
         }
-        float tmin, tmax;
-        if (a.tmin_arr) {
-            tmin = a.tmin_arr[i];
-            tmax = a.tmax_arr[i];
-        } else if (meta_depth(q.meta[i]) == 0) {
-            tmin = near_clip;
-            tmax = far_clip;
-        } else {
-            tmin = kEps;
-            tmax = kInf;
+        if (!__any(active)) break;
+        if (active && s.step<false, STATS>(a.nodes, a.tris, stk)) {
+            a.hits.t[i] = s.found ? s.best_t : -1.0f;
+            a.hits.prim[i] = s.best_prim;
+            a.hits.inst[i] = s.best_inst;
+            a.hits.b0[i] = s.bb0;
+            a.hits.b1[i] = s.bb1;
+            if (STATS) {
+                sn += s.n_nodes;
+                st += s.n_tris;
+            }
+            active = false;
         }
-        TraceResult r;
-        traverse<false, STATS>(a.nodes, a.tris, root, o, d, tmin, tmax, stk, r);
-        a.hits.t[i] = r.t;
-        a.hits.prim[i] = r.prim;
-        a.hits.inst[i] = r.inst;
-        a.hits.b0[i] = r.b0;
-        a.hits.b1[i] = r.b1;
-        if (STATS) { sn += r.nodes; st += r.tris; }
     }
     if (STATS) {
         atomicAdd(&a.counters->nodes_closest, sn);
@@ -326,40 +396,51 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
     unsigned long long sn = 0, st = 0;
     BatchScheduler sched(count);
     if (blockIdx.x >= sched.nb) return;
+    const ShadowQueue& q = a.shadows;
+    RayState s;
+    bool active = false;
+    uint32_t i = 0;
+    RayPool pool(sched, fetch, count);
     while (true) {
-        const uint32_t batch = sched.next(fetch);
-        if (batch >= sched.nb) break;
-        const uint32_t i = batch * (uint32_t)kTraceBlock + lane;
-        if (i >= count) continue;
-        const ShadowQueue& q = a.shadows;
-        const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
-        const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
-        const float tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
-        const float tmax = q.tmax[i];
-        uint32_t root = 0;
-        if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
-        TraceResult r;
-        const bool hit = traverse<true, STATS>(a.nodes, a.tris, root, o, d, tmin, tmax, stk, r);
-        if (a.image) {
-            if (!hit) {
-                float* px = a.image + 4u * (size_t)q.pixel[i];
-                atomicAdd(px + 0, q.wx[i]);
-                atomicAdd(px + 1, q.wy[i]);
-                atomicAdd(px + 2, q.wz[i]);
-                atomicAdd(px + 3, 1.0f);
-                // a training path has one shadow ray in flight, so this is its only writer
-                const uint32_t slot = q.slot ? q.slot[i] : kRcNoSlot;
-                if (slot != kRcNoSlot && a.slot_radiance) {
-                    float* sr = a.slot_radiance + 3 * (size_t)(slot & ~kRcRecorded);
-                    sr[0] += q.wx[i];
-                    sr[1] += q.wy[i];
-                    sr[2] += q.wz[i];
-                }
-            }
-        } else {
-            a.occluded[i] = hit ? 1u : 0u;
+        if (pool.refill(active)) {
+            i = pool.ray;
+            const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
+            const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
+            const float tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
+            const float tmax = q.tmax[i];
+            uint32_t root = 0;
+            if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
+            s.init(root, o, d, tmin, tmax);
+            active = true;
         }
-        if (STATS) { sn += r.nodes; st += r.tris; }
+        if (!__any(active)) break;
+        if (active && s.step<true, STATS>(a.nodes, a.tris, stk)) {
+            const bool hit = s.found;
+            if (a.image) {
+                if (!hit) {
+                    float* px = a.image + 4u * (size_t)q.pixel[i];
+                    atomicAdd(px + 0, q.wx[i]);
+                    atomicAdd(px + 1, q.wy[i]);
+                    atomicAdd(px + 2, q.wz[i]);
+                    atomicAdd(px + 3, 1.0f);
+                    // a training path has one shadow ray in flight, so this is its only writer
+                    const uint32_t slot = q.slot ? q.slot[i] : kRcNoSlot;
+                    if (slot != kRcNoSlot && a.slot_radiance) {
+                        float* sr = a.slot_radiance + 3 * (size_t)(slot & ~kRcRecorded);
+                        sr[0] += q.wx[i];
+                        sr[1] += q.wy[i];
+                        sr[2] += q.wz[i];
+                    }
+                }
+            } else {
+                a.occluded[i] = hit ? 1u : 0u;
+            }
+            if (STATS) {
+                sn += s.n_nodes;
+                st += s.n_tris;
+            }
+            active = false;
+        }
     }
     if (STATS) {
         atomicAdd(&a.counters->nodes_any, sn);
