@@ -104,6 +104,8 @@ struct RayState {
             const float sz = exp_scale((uint8_t)((q0.w >> 16) & 0xffu));
             const uint32_t imask = q0.w >> 24;
             uint32_t ihits = 0, thits = 0;
+            float near_t = kInf;
+            int near_k = 0;
 #pragma unroll
             for (int c = 0; c < 8; c++) {
                 const uint32_t meta = byte_of(q1.z, q1.w, c);
@@ -116,11 +118,25 @@ struct RayState {
                               decode_q(pz, (uint8_t)byte_of(q4.z, q4.w, c), sz));
                 float tn;
                 if (box_hit(lo, hi, o, inv, tmin, best_t, tn)) {
-                    if ((imask >> c) & 1u) ihits |= 1u << (c ^ oct);
-                    else thits |= (meta >> 5) << (meta & 31u);
+                    if ((imask >> c) & 1u) {
+                        ihits |= 1u << (c ^ oct);
+                        if (!ANY && tn < near_t) {
+                            near_t = tn;
+                            near_k = c ^ oct;
+                        }
+                    } else {
+                        thits |= (meta >> 5) << (meta & 31u);
+                    }
                 }
             }
             cur = make_uint2(q1.x, (ihits << 24) | imask);
+            // closest hit: descend into the nearest child first; the other
+            // hit children wait on the stack as one group
+            if (!ANY && (ihits & (ihits - 1u))) {
+                stk.store(sp, make_uint2(q1.x, ((ihits & ~(1u << near_k)) << 24) | imask));
+                sp++;
+                cur.y = (1u << (24 + near_k)) | imask;
+            }
             tri = make_uint2(q1.y, thits);
         } else {
             if (sp == 0) return true;
