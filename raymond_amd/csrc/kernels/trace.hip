@@ -120,7 +120,7 @@ struct RayState {
                 if (box_hit(lo, hi, o, inv, tmin, best_t, tn)) {
                     if ((imask >> c) & 1u) {
                         ihits |= 1u << (c ^ oct);
-                        if (!ANY && tn < near_t) {
+                        if (tn < near_t) {
                             near_t = tn;
                             near_k = c ^ oct;
                         }
@@ -130,9 +130,10 @@ struct RayState {
                 }
             }
             cur = make_uint2(q1.x, (ihits << 24) | imask);
-            // closest hit: descend into the nearest child first; the other
-            // hit children wait on the stack as one group
-            if (!ANY && (ihits & (ihits - 1u))) {
+            // descend into the nearest child first (closest hit: best_t
+            // shrinks sooner; any hit: near occluders are found sooner); the
+            // other hit children wait on the stack as one group
+            if (ihits & (ihits - 1u)) {
                 stk.store(sp, make_uint2(q1.x, ((ihits & ~(1u << near_k)) << 24) | imask));
                 sp++;
                 cur.y = (1u << (24 + near_k)) | imask;
