@@ -37,7 +37,10 @@ def copy_probe(mib):
     dev.close()
 
 
-def workload_name(graphs, W, H, D, sort):
+def workload_name(graphs, W, H, D, sort, cornell=False):
+    if cornell:
+        return ("Cornell box (config 1): 36 triangles, emitter 15, %dx%d, depth %d, roulette from bounce 3, "
+                "plain tracer, 1 spp per step" % (W, H, D))
     if graphs:
         return ("shader graphs (config 3): Cornell-box geometry, 32 random 20-60 node Cycles graphs (seed 3) "
                 "assigned per triangle, %dx%d, depth %d, plain tracer, 1 spp per step%s" %
@@ -222,8 +225,9 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
-    ap.add_argument("--workload", choices=["spheres", "graphs", "cache", "batch"], default="spheres",
-                    help="spheres: 5.2M-triangle sphere field; graphs: 32 random node-graph materials; "
+    ap.add_argument("--workload", choices=["spheres", "cornell", "graphs", "cache", "batch"], default="spheres",
+                    help="spheres: 5.2M-triangle sphere field; cornell: the 36-triangle Cornell box at 512^2; "
+                         "graphs: 32 random node-graph materials; "
                          "cache: learned radiance cache against the plain tracer on the Cornell box; "
                          "batch: 64 uneven Cornell-box jobs in one bounce loop")
     ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
@@ -259,10 +263,22 @@ def main():
         copy_probe(args.copy_probe)
     W, H = args.width, args.height
     graphs = args.workload == "graphs"
-    D = args.depth or (8 if graphs else 12)
+    cornell = args.workload == "cornell"
+    if cornell and (W, H) == (1920, 1080):
+        W, H = 512, 512
+    D = args.depth or (8 if graphs or cornell else 12)
     sort = args.material_sort if args.material_sort >= 0 else int(graphs)
+    # configuration 1 runs roulette from bounce 3; the sphere field and graph
+    # scenes the reference's way, from the first bounce
+    rr_depth = 3 if cornell else 0
     t0 = time.time()
-    desc = scenes.shader_graph_scene(W, H) if graphs else scenes.sphere_field(W, H)
+    if graphs:
+        desc = scenes.shader_graph_scene(W, H)
+    elif cornell:
+        desc = scenes.cornell_box(W, H)
+        desc.envmap_resolution = 64  # black world
+    else:
+        desc = scenes.sphere_field(W, H)
     dev = api.Device(0)
     # the graph scene inlines its materials (one compile, cached; faster shading)
     inline = args.inline_materials if args.inline_materials >= 0 else (1 if graphs else -1)
@@ -276,7 +292,7 @@ def main():
 
     spl = args.spp_per_launch
     r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
-    r.set_uniforms(timing=0 if args.graph else 1, rr_depth=0, material_sort=sort, use_graph=args.graph)
+    r.set_uniforms(timing=0 if args.graph else 1, rr_depth=rr_depth, material_sort=sort, use_graph=args.graph)
     if args.warmup:
         r.execute(args.warmup)
     # the timed steps run as one regenerating stream of `steps` samples per pixel
@@ -311,7 +327,7 @@ def main():
     # closest rays per launch
     traffic = None
     traffic_file = os.path.join(HERE, "profiles", "trace_closest_c2_traffic.json")
-    if not graphs and os.path.exists(traffic_file):
+    if not graphs and not cornell and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
         # one closest-hit launch per bounce-loop iteration
@@ -333,7 +349,7 @@ def main():
             pix = None
             frames = args.cpu_frames
             sample = "%d full %dx%d frames (%d bench steps), 1 spp each, depth %d" % (frames, W, H, frames, D)
-        _, res = orc.render(W, H, D, frames=frames, frame_begin=10_000, pixels=pix, threads=threads)
+        _, res = orc.render(W, H, D, frames=frames, frame_begin=10_000, pixels=pix, threads=threads, rr_depth=rr_depth)
         cpu = {"value": res.total_rays / res.seconds / 1e6, "unit": "Mrays/s", "cores": int(res.threads_used),
                "kind": "port", "sample": sample, "seconds": round(res.seconds, 2)}
         print("cpu oracle: %.3f Mrays/s on %d threads (%.1fs)" % (cpu["value"], cpu["cores"], res.seconds),
@@ -353,7 +369,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": workload_name(graphs, W, H, D, sort),
+            "workload": workload_name(graphs, W, H, D, sort, cornell),
             "width": W, "height": H, "max_depth": D, "queue_frames": spl, "triangles": int(sst.triangles),
             "iterations": int(totals["iters"]), "material_sort": bool(sort),
         },

@@ -138,7 +138,7 @@ struct ShadeOut {
     float feat[kRcFeatureFloats];
 };
 
-template <bool BATCHED>
+template <bool BATCHED, bool CACHED>
 __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs& a, uint32_t i, ShadeOut& out) {
     // batched jobs: the path's job supplies the lights and the depth limit
     SceneView job_ctx = a.ctx;
@@ -164,7 +164,7 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
 
     // a training path of the radiance cache also sums its own contributions
     const CacheView& cv = a.cache;
-    const uint32_t slot = cv.slots ? cv.slots[i] : kRcNoSlot;
+    const uint32_t slot = (CACHED && cv.slots) ? cv.slots[i] : kRcNoSlot;
     const bool training = slot != kRcNoSlot;
     float* slot_rad = training ? cv.slot_radiance + 3 * (size_t)(slot & ~kRcRecorded) : nullptr;
     vec3 rad(0.0f);
@@ -222,7 +222,7 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
     // Radiance cache: the first vertex after a diffuse bounce ends an
     // ordinary path with a lookup; a training path records that vertex (its
     // target is the radiance the rest of the path gathers) and goes on.
-    if (cv.enabled && depth >= 1u && (flags & RayDiffuse) && shading.material.alpha >= 1.0f &&
+    if (CACHED && cv.enabled && depth >= 1u && (flags & RayDiffuse) && shading.material.alpha >= 1.0f &&
         !shading.material.is_delta()) {
         if (!training) {
             out.cache_kind = 1u;
@@ -292,7 +292,7 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
 // counters once per workgroup (jobs beyond the LDS table count directly).
 constexpr uint32_t kJobsLds = 256;
 
-template <bool BATCHED>
+template <bool BATCHED, bool CACHED>
 __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArgs& a) {
     __shared__ uint32_t job_ended[kJobsLds];
     constexpr bool batched = BATCHED;
@@ -306,18 +306,21 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
     for (uint32_t base = blockIdx.x * kShadeBlock; base < count; base += stride) {
         const uint32_t k = base + threadIdx.x;
         ShadeOut out;
-        if (k < count) shade_path<BATCHED>(a, a.order ? a.order[k] : k, out);
+        if (k < count) shade_path<BATCHED, CACHED>(a, a.order ? a.order[k] : k, out);
         if (batched && k < count && !out.next) {
             if (out.job < kJobsLds) atomicAdd(&job_ended[out.job], 1u);
             else atomicAdd(&a.batch.done[out.job], 1u);
         }
+        constexpr int kStreams = CACHED ? 4 : 2;
         uint32_t* const ctr[4] = {&a.counters->it[iter].shadows, &a.counters->it[iter + 1].appended,
                                   &a.counters->cache_queries, &a.counters->train_records};
         const bool want[4] = {out.shadow, out.next, out.cache_kind == 1u, out.cache_kind == 2u};
-        uint32_t at[4];
-        block_append<4>(ctr, want, at);
+        uint32_t at[4] = {0u, 0u, 0u, 0u};
+        block_append<kStreams>(reinterpret_cast<uint32_t* const(&)[kStreams]>(ctr),
+                               reinterpret_cast<const bool(&)[kStreams]>(want),
+                               reinterpret_cast<uint32_t(&)[kStreams]>(at));
         const uint32_t sslot = at[0], nslot = at[1];
-        if (out.cache_kind == 1u && at[2] < a.cache.query_capacity) {
+        if (CACHED && out.cache_kind == 1u && at[2] < a.cache.query_capacity) {
             const RcQueryQueue& q = a.cache.queries;
             const uint32_t j = at[2];
             q.pixel[j] = out.pixel;
@@ -326,7 +329,7 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
             q.tz[j] = out.c_t.z;
 #pragma unroll
             for (int f = 0; f < kRcFeatureFloats; f++) q.feat[(size_t)f * a.cache.query_capacity + j] = out.feat[f];
-        } else if (out.cache_kind == 2u && at[3] < a.cache.train_capacity) {
+        } else if (CACHED && out.cache_kind == 2u && at[3] < a.cache.train_capacity) {
             const RcTrainQueue& q = a.cache.train;
             const uint32_t j = at[3];
             q.slot[j] = out.slot & ~kRcRecorded;
@@ -351,7 +354,7 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
             a.shadows.wy[sslot] = out.s_weight.y;
             a.shadows.wz[sslot] = out.s_weight.z;
             a.shadows.pixel[sslot] = out.pixel;
-            if (a.shadows.slot) a.shadows.slot[sslot] = out.slot;
+            if (CACHED && a.shadows.slot) a.shadows.slot[sslot] = out.slot;
         }
         if (out.next) {
             a.next.ox[nslot] = out.n_origin.x;
@@ -367,7 +370,7 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
             a.next.seed[nslot] = out.seed;
             a.next.meta[nslot] = out.n_meta;
             a.next.pixel[nslot] = out.pixel;
-            if (a.cache.slots_next) a.cache.slots_next[nslot] = out.slot;
+            if (CACHED && a.cache.slots_next) a.cache.slots_next[nslot] = out.slot;
         }
     }
     if (batched) {
@@ -377,12 +380,22 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
     }
 }
 
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) { shade_queue<false>(a); }
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) { shade_queue<false, false>(a); }
+
+#if defined(RM_CACHE_KERNEL)
+// Renderers with a radiance cache: lookups, training paths and their
+// radiance sums.  Compiled only into the modules of scenes a cache renders.
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_cache_kernel(ShadeArgs a) {
+    shade_queue<false, true>(a);
+}
+#endif
 
 #if defined(RM_BATCH_KERNEL)
 // Batched jobs (a.batch.jobs non-null): per-job lights, depth and path counts.
 // Compiled only into the modules of batches.
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_batch_kernel(ShadeArgs a) { shade_queue<true>(a); }
+extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_batch_kernel(ShadeArgs a) {
+    shade_queue<true, false>(a);
+}
 #endif
 
 // 64 world-shader samples per texel; writes the finest mip level in 2x2-quad

@@ -65,6 +65,7 @@ struct JitModule {
     hipModule_t module = nullptr;
     hipFunction_t shade = nullptr;
     hipFunction_t shade_batch = nullptr;  // present in modules compiled with -DRM_BATCH_KERNEL
+    hipFunction_t shade_cache = nullptr;  // present in modules compiled with -DRM_CACHE_KERNEL
     hipFunction_t envmap_build = nullptr;
     ~JitModule() {
         if (module) (void)hipModuleUnload(module);
@@ -127,6 +128,7 @@ struct rm_scene {
     bool materials_built = false;
     bool envmap_built = false;
     rm::JitModule jit;
+    rm::JitModule jit_cache;      // the library again with the radiance-cache shading kernel (on demand)
     std::string material_source;  // as compiled (batches recompile it with the batch kernel)
     rm_scene_stats stats{};
 };

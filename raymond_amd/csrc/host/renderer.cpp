@@ -125,6 +125,7 @@ struct StageTimer {
 static constexpr uint32_t kIterChunk = 8;
 
 void renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode) {
+    if (c) scene_cache_module(r->scene);
     r->cache = c;
     r->cache_mode = c ? mode : 0u;
     if (!c) return;
@@ -215,7 +216,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     const int tblocks = cus * dev->trace_blocks_per_cu;
     const int ablocks = cus * dev->any_blocks_per_cu;
     BatchRun* br = r->batch;
-    hipFunction_t shade_fn = br ? s->jit.shade_batch : s->jit.shade;
+    hipFunction_t shade_fn = br ? s->jit.shade_batch : (r->cache_mode ? s->jit_cache.shade_cache : s->jit.shade);
     RM_REQUIRE(shade_fn != nullptr, "material module lacks the batch shading kernel");
     int shade_bpc = 0;
     RM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&shade_bpc, shade_fn, 256, 0));

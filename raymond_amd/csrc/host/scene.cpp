@@ -260,14 +260,12 @@ void scene_build_accel(rm_scene* s, int threads) {
     s->stats.bvh_build_seconds = b.build_seconds;
 }
 
-void scene_compile_materials(rm_scene* s, const char* src, size_t len, const char* options, std::string& log) {
-    RM_REQUIRE(src != nullptr, "null material source");
-    const auto t0 = std::chrono::steady_clock::now();
+// Compiles (or takes from the code-object cache) and loads one module.
+static void load_module(JitModule& j, const std::string& src, const char* options, std::string& log) {
     std::vector<char> code;
     bool hit = false;
-    if (!jit_compile(std::string(src, len), options ? options : "", code, log, hit))
+    if (!jit_compile(src, options ? options : "", code, log, hit))
         throw HipError{RM_ERROR_COMPILE, "material compilation failed:\n" + log};
-    JitModule& j = s->jit;
     if (j.module) {
         (void)hipModuleUnload(j.module);
         j.module = nullptr;
@@ -275,11 +273,34 @@ void scene_compile_materials(rm_scene* s, const char* src, size_t len, const cha
     RM_HIP(hipModuleLoadData(&j.module, code.data()));
     RM_HIP(hipModuleGetFunction(&j.shade, j.module, "rm_shade_kernel"));
     RM_HIP(hipModuleGetFunction(&j.envmap_build, j.module, "rm_envmap_build_kernel"));
-    if (hipModuleGetFunction(&j.shade_batch, j.module, "rm_shade_batch_kernel") != hipSuccess) {
-        (void)hipGetLastError();
-        j.shade_batch = nullptr;
-    }
+    auto optional = [&](hipFunction_t& f, const char* name) {
+        if (hipModuleGetFunction(&f, j.module, name) != hipSuccess) {
+            (void)hipGetLastError();
+            f = nullptr;
+        }
+    };
+    optional(j.shade_batch, "rm_shade_batch_kernel");
+    optional(j.shade_cache, "rm_shade_cache_kernel");
+}
+
+void scene_cache_module(rm_scene* s) {
+    if (s->jit_cache.shade_cache) return;
+    RM_REQUIRE(s->materials_built, "materials must be compiled before a cache is attached");
+    std::string log;
+    load_module(s->jit_cache, s->material_source, "-DRM_CACHE_KERNEL", log);
+    RM_REQUIRE(s->jit_cache.shade_cache != nullptr, "cache shading kernel missing from the module");
+}
+
+void scene_compile_materials(rm_scene* s, const char* src, size_t len, const char* options, std::string& log) {
+    RM_REQUIRE(src != nullptr, "null material source");
+    const auto t0 = std::chrono::steady_clock::now();
+    load_module(s->jit, std::string(src, len), options, log);
     s->material_source.assign(src, len);
+    if (s->jit_cache.module) {
+        (void)hipModuleUnload(s->jit_cache.module);
+        s->jit_cache.module = nullptr;
+        s->jit_cache.shade_cache = nullptr;
+    }
     s->materials_built = true;
     s->stats.jit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -17,6 +17,8 @@ void scene_set_camera(rm_scene* s, const rm_camera_desc* c);
 void scene_build_accel(rm_scene* s, int threads);
 void scene_compile_materials(rm_scene* s, const char* src, size_t len, const char* options, std::string& log);
 void scene_build_envmap(rm_scene* s);
+// Loads the scene's material library with the radiance-cache shading kernel.
+void scene_cache_module(rm_scene* s);
 void trace_closest_raw(rm_scene* s, const float* rays, uint32_t n, float* t, uint32_t* inst, uint32_t* prim,
                        float* bary);
 void trace_any_raw(rm_scene* s, const float* rays, uint32_t n, uint32_t* occluded);
