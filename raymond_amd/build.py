@@ -87,9 +87,11 @@ def warm_jit_cache(verbose=False, big=False):
             "src = sys.stdin.read(); t = time.time(); api.jit_precompile(src, sys.argv[1]); "
             "print(round(time.time() - t, 1))" % root)
     # batches compile their library again with the batch shading kernel
+    by_name = dict(jobs)
     jobs = [(name, src, "") for name, src in jobs] + [
-        (jobs[0][0] + ", batch kernel", jobs[0][1], "-DRM_BATCH_KERNEL"),
-        (jobs[0][0] + ", cache kernel", jobs[0][1], "-DRM_CACHE_KERNEL")]
+        ("cornell, batch kernel", by_name["cornell"], "-DRM_BATCH_KERNEL"),
+        ("lamp box, batch kernel", by_name["lamp box"], "-DRM_BATCH_KERNEL"),
+        ("cornell, cache kernel", by_name["cornell"], "-DRM_CACHE_KERNEL")]
     procs = []
     for name, src, options in jobs:
         p = subprocess.Popen([sys.executable, "-c", code, options], env=env, stdin=subprocess.PIPE,

@@ -401,6 +401,65 @@ def tessellated_cornell_box(width, height, triangles):
     return d
 
 
+def light_info(shader_index, casts_shadows=True, uses_mis=False):
+    return api.LightInfo(shader_index, int(casts_shadows), int(uses_mis))
+
+
+def area_light(info, center, size, color, circular=False, rotation=None):
+    """Lamp of `size` (x, y) around `center`; uv in [-0.5, 0.5]^2 maps through
+    the 3x4 affine rows.  It emits toward -normal, the -z axis of `rotation`."""
+    R = np.eye(3) if rotation is None else np.asarray(rotation, np.float64)[:3, :3]
+    M = np.zeros((3, 4))
+    M[:, 0], M[:, 1], M[:, 2] = R[:, 0] * size[0], R[:, 1] * size[1], R[:, 2]
+    M[:, 3] = center
+    return api.AreaLight(info, (C.c_float * 12)(*M.reshape(-1)), (C.c_float * 3)(*color), int(circular))
+
+
+def point_light(info, location, radius, color):
+    return api.PointLight(info, (C.c_float * 3)(*location), radius, (C.c_float * 3)(*color))
+
+
+def sun_light(info, direction, cos_angle, color):
+    """`direction` points from the scene toward the sun."""
+    d = np.asarray(direction, np.float64)
+    d /= np.linalg.norm(d)
+    return api.SunLight(info, (C.c_float * 3)(*d), cos_angle, (C.c_float * 3)(*color))
+
+
+def spot_light(info, location, direction, radius, color, spot_size, spot_blend):
+    """`direction` points from the lit region back toward the lamp (the
+    attenuation is its dot product with the receiver-to-lamp direction);
+    `spot_size` is the cosine of the cone's half angle, `spot_blend` the
+    smoothing width in cosine."""
+    d = np.asarray(direction, np.float64)
+    d /= np.linalg.norm(d)
+    return api.SpotLight(info, (C.c_float * 3)(*location), (C.c_float * 3)(*d), radius, (C.c_float * 3)(*color),
+                         spot_size, spot_blend)
+
+
+def lamp_box(width=64, height=64):
+    """The Cornell box lit by one lamp of every analytic type instead of its
+    emissive quad: a rectangular and a circular area lamp under the ceiling,
+    a point lamp with a radius, a spot lamp and a sun shining in through the
+    open front.  Lamps use two emission shaders; some take part in MIS."""
+    d = cornell_box(width, height)
+    d.name = "lamp box"
+    d.entities = d.entities[:3]
+    d.shapes = d.shapes[:3]
+    d.light_materials = [background_world("world", (0.0, 0.0, 0.0), 1.0),
+                         emission_material("warm", (1.0, 0.85, 0.6), 4.0),
+                         emission_material("cool", (0.6, 0.8, 1.0), 2.0)]
+    d.area_lights = [
+        area_light(light_info(1, uses_mis=True), (0.3, 0.5, 0.98), (0.25, 0.4), (1.0, 1.0, 1.0)),
+        area_light(light_info(2), (0.72, 0.35, 0.97), (0.2, 0.2), (1.5, 1.5, 1.5), circular=True),
+    ]
+    d.point_lights = [point_light(light_info(1, uses_mis=True), (0.5, 0.3, 0.8), 0.05, (3.0, 3.0, 3.0))]
+    d.sun_lights = [sun_light(light_info(2), (0.2, -1.0, 0.6), 0.995, (0.8, 0.8, 0.8))]
+    d.spot_lights = [spot_light(light_info(1, casts_shadows=True), (0.25, 0.25, 0.9), (0.3, 0.2, 1.0), 0.02,
+                                (6.0, 6.0, 6.0), 0.85, 0.05)]
+    return d
+
+
 def batch_jobs(n=64, seed=4):
     """Config 4: n Cornell-box jobs with triangle counts log-uniform in
     [1k, 2M], resolutions from {256, 512, 768, 1024}^2, 16-256 spp and depth
@@ -728,6 +787,7 @@ def bundled_material_sources(big=False):
     for name, desc in [("cornell", cornell_box(64, 64)),
                        ("small spheres", sphere_field(96, 54, grid=2, level=3)),
                        ("graphs, 8 materials", shader_graph_scene(48, 48, materials=8, seed=3)),
+                       ("lamp box", lamp_box(48, 48)),
                        ("sphere field", sphere_field(1920, 1080))]:
         out[name] = desc.material_source()
     if big:

@@ -242,3 +242,55 @@ def test_graph_replay_matches_direct_launches(cornell):
     assert out[0][0] == out[1][0]
     scale = max(np.abs(out[0][1]).max(), 1e-6)
     assert np.abs(out[0][1] - out[1][1]).max() <= 1e-5 * scale
+
+
+@pytest.fixture(scope="module")
+def lamps(device):
+    """The Cornell box lit by a rectangular and a circular area lamp, a point
+    lamp with a radius, a spot lamp and a sun, through two lamp shaders."""
+    desc = scenes.lamp_box(48, 48)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc)
+    orc = binding.OracleScene(desc)
+    orc.build_envmap()
+    return desc, scene, orc
+
+
+@pytest.mark.parametrize("mode", [2, 1, 0])
+def test_render_lamps_per_sample(lamps, mode):
+    """Every analytic lamp type (sampling, lamp shader, shadow flag, MIS flag)
+    under MIS, light sampling only and BSDF sampling only."""
+    desc, scene, orc = lamps
+    W, H = 48, 48
+    r = api.Renderer(scene, W, H, max_depth=6)
+    r.set_uniforms(sampling_mode=mode)
+    prev = np.zeros((H, W, 4), np.float32)
+    for frame in range(2):
+        st = r.execute(1)
+        img = r.image()
+        sample = img - prev
+        prev = img
+        ref, res = orc.render(W, H, 6, frames=1, frame_begin=frame, sampling_mode=mode)
+        assert [st.rays[d] for d in range(6)] == [res.rays[d] for d in range(6)]
+        assert [st.shadow_rays[d] for d in range(5)] == [res.shadow_rays[d] for d in range(5)]
+        scale = max(np.abs(ref).max(), 1e-6)
+        assert np.abs(sample - ref).max() <= 1e-4 * scale + 1e-6
+        if mode != 0:
+            assert ref.max() > 0.0
+
+
+def test_batch_of_lamp_scenes_matches_solo(device, lamps):
+    """Each job of a batch keeps its own lamps."""
+    desc, scene, orc = lamps
+    d2 = scenes.lamp_box(32, 24)
+    d2.point_lights = []
+    d2.envmap_resolution = 64
+    s2 = scenes.build_device_scene(device, d2)
+    jobs = [(scene, 48, 48, 2, 6, 0), (s2, 32, 24, 3, 5, 0)]
+    b = api.Batch(device, jobs)
+    b.render(queue_paths=4096)
+    for j, (s, w, h, spp, depth, _) in enumerate(jobs):
+        r = api.Renderer(s, w, h, max_depth=depth)
+        r.execute(spp)
+        ref = r.image()
+        assert np.abs(b.image(j) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-6), j
