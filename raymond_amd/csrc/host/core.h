@@ -148,6 +148,8 @@ struct rm_renderer {
     rm_cache* cache = nullptr;
     uint32_t cache_mode = 0;           // 0 off, 1 query + train, 2 query only
     size_t slot_capacity = 0;          // training paths per stream
+    size_t query_capacity = 0;         // lookups per stream
+    uint32_t cache_frames = 0;         // samples rendered since the cache was attached
     rm::DevBuf slot_buf, slot_rad_buf, query_buf, train_buf, target_buf, shadow_slot_buf;
     rm::CacheView cv{};
     rm::vec3 world_lo, world_hi;

@@ -2,6 +2,7 @@
 // raw-batch entry points.
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "rcache_host.h"
@@ -24,6 +25,21 @@ struct InitRng {
     float uniform(float a) { return a * (2.0f * (float)((next() >> 40) * (1.0 / 16777216.0)) - 1.0f); }
 };
 
+// IEEE binary16 bit pattern to float (exact).
+float half_bits_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0u) {
+        v = std::ldexp((float)m, -24);  // zero and subnormals
+    } else if (e == 31u) {
+        v = m ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+    } else {
+        v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+    }
+    return sign ? -v : v;
+}
+
 // Feature-major copy of n x k row-major host rows.
 std::vector<float> to_columns(const float* rows, uint32_t n, int k) {
     std::vector<float> cols((size_t)n * k);
@@ -33,7 +49,7 @@ std::vector<float> to_columns(const float* rows, uint32_t n, int k) {
 }
 
 void zero_grads(rm_cache* c, hipStream_t s) {
-    RM_HIP(launch_rc_fill(c->t_grad.as<float>(), 0.0f, c->table_floats, s));
+    RM_HIP(launch_rc_fill(c->t_grad.as<float>(), 0.0f, c->table_floats / 2, s));  // FP16 pairs
     RM_HIP(launch_rc_fill(c->w_grad.as<float>(), 0.0f, kRcMlpParams, s));
 }
 
@@ -49,7 +65,7 @@ void ensure_partial(rm_cache* c, uint32_t batch) {
 void cache_backward(rm_cache* c, const RcSamples& smp, uint32_t batch, hipStream_t s) {
     ensure_partial(c, batch);
     RcGrads g;
-    g.table = c->t_grad.as<float>();
+    g.table = c->t_grad.as<uint32_t>();
     g.mlp_partial = c->partial.as<float>();
     g.loss = c->loss.as<float>();
     RM_HIP(launch_rc_train(cache_params(c), smp, batch, g, 1.0f / (float)batch, s));
@@ -102,7 +118,7 @@ void cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache* c) {
     c->t_master.alloc(c->table_floats * 4);
     c->t_m.alloc(c->table_floats * 4);
     c->t_v.alloc(c->table_floats * 4);
-    c->t_grad.alloc(c->table_floats * 4);
+    c->t_grad.alloc(c->table_floats * 2);
     c->t_half.alloc(c->table_floats * 2);
     c->w_master.alloc(kRcMlpParams * 4);
     c->w_m.alloc(kRcMlpParams * 4);
@@ -150,7 +166,8 @@ float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_lo
     a.master = c->t_master.as<float>();
     a.m = c->t_m.as<float>();
     a.v = c->t_v.as<float>();
-    a.grad = c->t_grad.as<float>();
+    a.grad = nullptr;
+    a.grad_h2 = c->t_grad.as<uint32_t>();
     a.half_out = c->t_half.as<uint32_t>();
     a.n = (uint32_t)c->table_floats;
     RM_HIP(launch_rc_adam_table(a, s));
@@ -235,10 +252,12 @@ void cache_gradients_raw(rm_cache* c, const float* features, const float* target
     RM_HIP(launch_rc_fill(c->loss.as<float>(), 0.0f, 1, s));
     cache_backward(c, smp, n, s);
     if (grad_mlp) RM_HIP(hipMemcpyAsync(grad_mlp, c->w_grad.ptr, kRcMlpParams * 4, hipMemcpyDeviceToHost, s));
-    if (grad_table) RM_HIP(hipMemcpyAsync(grad_table, c->t_grad.ptr, c->table_floats * 4, hipMemcpyDeviceToHost, s));
+    std::vector<uint16_t> bits(grad_table ? c->table_floats : 0);
+    if (grad_table) RM_HIP(hipMemcpyAsync(bits.data(), c->t_grad.ptr, c->table_floats * 2, hipMemcpyDeviceToHost, s));
     if (loss) RM_HIP(hipMemcpyAsync(loss, c->loss.ptr, 4, hipMemcpyDeviceToHost, s));
     zero_grads(c, s);
     RM_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < bits.size(); i++) grad_table[i] = half_bits_to_float(bits[i]) / kRcTableGradScale;
 }
 
 void cache_train_raw(rm_cache* c, const float* features, const float* targets, uint32_t n, uint32_t steps,

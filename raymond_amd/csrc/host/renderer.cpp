@@ -123,11 +123,13 @@ struct StageTimer {
 // device counters, and the host checks for completion once every
 // kIterChunk iterations.
 static constexpr uint32_t kIterChunk = 8;
+static constexpr uint32_t kCacheStreamMax = 8;
 
 void renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode) {
     if (c) scene_cache_module(r->scene);
     r->cache = c;
     r->cache_mode = c ? mode : 0u;
+    r->cache_frames = 0;
     if (!c) return;
     const std::vector<BuildTri> tris = world_triangles(r->scene);
     vec3 lo(kInf), hi(-kInf);
@@ -150,7 +152,12 @@ static void prepare_cache(rm_renderer* r, uint32_t samples) {
     const uint32_t shift = c->desc.train_shift;
     const size_t total = (size_t)r->width * r->height * samples;
     const size_t slots = train ? ((total + (1ull << shift) - 1) >> shift) + 1 : 0;
-    if (!r->query_buf.ptr) r->query_buf.alloc(cap * (4 + kRcFeatureFloats) * 4);
+    // every path of the stream looks up at most once
+    const size_t queries = std::max<size_t>(cap, (size_t)r->width * r->height * samples);
+    if (queries > r->query_capacity) {
+        r->query_capacity = queries;
+        r->query_buf.alloc(queries * (4 + kRcFeatureFloats) * 4);
+    }
     if (train && slots > r->slot_capacity) {
         r->slot_capacity = slots;
         r->slot_rad_buf.alloc(slots * 3 * 4);
@@ -164,19 +171,20 @@ static void prepare_cache(rm_renderer* r, uint32_t samples) {
     cv = CacheView{};
     cv.enabled = 1u;
     cv.train_shift = shift;
-    cv.query_capacity = (uint32_t)cap;
+    cv.query_capacity = (uint32_t)r->query_capacity;
     cv.train_capacity = (uint32_t)r->slot_capacity;
     cv.slot_capacity = (uint32_t)r->slot_capacity;
     cv.bounds_min = r->world_lo;
     const vec3 ext = r->world_hi - r->world_lo;
     cv.bounds_inv_extent = vec3(1.0f / max_(ext.x, 1e-6f), 1.0f / max_(ext.y, 1e-6f), 1.0f / max_(ext.z, 1e-6f));
+    const size_t qc = r->query_capacity;
     uint32_t* qu = r->query_buf.as<uint32_t>();
     float* qf = r->query_buf.as<float>();
     cv.queries.pixel = qu;
-    cv.queries.tx = qf + cap;
-    cv.queries.ty = qf + 2 * cap;
-    cv.queries.tz = qf + 3 * cap;
-    cv.queries.feat = qf + 4 * cap;
+    cv.queries.tx = qf + qc;
+    cv.queries.ty = qf + 2 * qc;
+    cv.queries.tz = qf + 3 * qc;
+    cv.queries.feat = qf + 4 * qc;
     if (train) {
         const size_t sc = r->slot_capacity;
         float* tf = r->train_buf.as<float>();
@@ -406,7 +414,40 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         if (br) batch_poll(br, st);
         if (probe.generated >= total && probe.appended == 0) break;
     }
+    uint32_t cache_ctr[2] = {0, 0};  // lookups, training vertices
     if (cached) {
+        RM_HIP(hipMemcpyAsync(cache_ctr, &cnt->cache_queries, 8, hipMemcpyDeviceToHost, st));
+        RM_HIP(hipStreamSynchronize(st));
+        rm_cache* c = r->cache;
+        c->queries += cache_ctr[0];
+        const uint32_t records = std::min<uint32_t>(cache_ctr[1], r->cv.train_capacity);
+        if (cache_train && records > 0) {
+            // targets of this stream's training vertices, then Adam steps on
+            // them (steps_per_launch per sample of the stream), before the
+            // stream's lookups so that they see what the stream taught
+            std::vector<std::pair<hipEvent_t, hipEvent_t>> t_train;
+            hipEvent_t et = tm.begin();
+            RM_HIP(launch_rc_targets(r->cv.train, r->cv.slot_radiance, &cnt->train_records, r->cv.train_capacity,
+                                     r->target_buf.as<float>(), st));
+            RcSamples smp{};
+            smp.feat = r->cv.train.feat;
+            smp.target = r->target_buf.as<float>();
+            smp.stride = r->cv.train_capacity;
+            smp.count = records;
+            smp.perm_mul = 2147483647u;  // prime, so a permutation of any smaller count
+            const uint32_t batch = std::min<uint32_t>(c->desc.batch_size, records);
+            const uint32_t steps = c->desc.steps_per_launch * samples;
+            for (uint32_t k = 0; k < steps; k++) {
+                smp.first = (uint32_t)(c->steps * 7919u) + k * batch;
+                cache_step(c, smp, batch, false);
+            }
+            tm.end(et, t_train);
+            c->records += records;
+            if (tm.on) {
+                RM_HIP(hipStreamSynchronize(st));
+                c->ms_train += StageTimer::sum(t_train);
+            }
+        }
         // the lookups of the whole stream in one batch
         RcInferArgs ia{};
         ia.p = cache_params(r->cache);
@@ -417,6 +458,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         hipEvent_t ei = tm.begin();
         RM_HIP(launch_rc_infer(ia, cus * 8, st));
         tm.end(ei, t_infer);
+        r->cache_frames += samples;
     }
     if (r->u.accumulate_f64) {
         if (!r->image64_buf.ptr) {
@@ -429,39 +471,8 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     RM_HIP(hipMemcpyAsync(host_it.data(), cnt->it, iter * sizeof(IterCounters), hipMemcpyDeviceToHost, st));
     unsigned long long tstat[4] = {0, 0, 0, 0};
     RM_HIP(hipMemcpyAsync(tstat, &cnt->nodes_closest, sizeof(tstat), hipMemcpyDeviceToHost, st));
-    uint32_t cache_ctr[2] = {0, 0};  // lookups, training vertices
-    RM_HIP(hipMemcpyAsync(cache_ctr, &cnt->cache_queries, 8, hipMemcpyDeviceToHost, st));
     RM_HIP(hipStreamSynchronize(st));
-    if (cached) {
-        rm_cache* c = r->cache;
-        c->queries += cache_ctr[0];
-        if (tm.on) c->ms_infer += StageTimer::sum(t_infer);
-        const uint32_t records = std::min<uint32_t>(cache_ctr[1], r->cv.train_capacity);
-        if (cache_train && records > 0) {
-            // targets of this stream's training vertices, then Adam steps on them
-            std::vector<std::pair<hipEvent_t, hipEvent_t>> t_train;
-            hipEvent_t et = tm.begin();
-            RM_HIP(launch_rc_targets(r->cv.train, r->cv.slot_radiance, &cnt->train_records, r->cv.train_capacity,
-                                     r->target_buf.as<float>(), st));
-            RcSamples smp{};
-            smp.feat = r->cv.train.feat;
-            smp.target = r->target_buf.as<float>();
-            smp.stride = r->cv.train_capacity;
-            smp.count = records;
-            smp.perm_mul = 2147483647u;  // prime, so a permutation of any smaller count
-            const uint32_t batch = std::min<uint32_t>(c->desc.batch_size, records);
-            for (uint32_t k = 0; k < c->desc.steps_per_launch; k++) {
-                smp.first = (uint32_t)(c->steps * 7919u) + k * batch;
-                cache_step(c, smp, batch, false);
-            }
-            tm.end(et, t_train);
-            c->records += records;
-            if (tm.on) {
-                RM_HIP(hipStreamSynchronize(st));
-                c->ms_train += StageTimer::sum(t_train);
-            }
-        }
-    }
+    if (cached && tm.on) r->cache->ms_infer += StageTimer::sum(t_infer);
     r->frame_index += samples;
     if (stats) {
         for (uint32_t i = 0; i < iter; i++) {
@@ -507,11 +518,18 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     RM_REQUIRE(s->envmap_built, "envmap not built");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const auto t0 = std::chrono::steady_clock::now();
-    // bound a stream so its iteration count stays within kMaxIters; a
-    // training cache learns between streams of one queue's worth of samples
-    const uint32_t max_stream = r->cache_mode != 0u ? r->spl : 512;
+    // bound a stream so its iteration count stays within kMaxIters.  With a
+    // radiance cache, streams double in length from one sample to
+    // kCacheStreamMax as the cache sees more samples: the cache learns
+    // between streams, and a longer stream spreads the drain of its last
+    // (training) paths over more samples.
     uint32_t left = samples;
     while (left > 0) {
+        uint32_t max_stream = 512;
+        if (r->cache_mode != 0u) {
+            max_stream = 1;
+            while (max_stream * 2 <= r->cache_frames && max_stream * 2 <= kCacheStreamMax) max_stream *= 2;
+        }
         const uint32_t n = left < max_stream ? left : max_stream;
         render_stream(r, n, stats);
         left -= n;

@@ -17,6 +17,7 @@
 
 namespace rm {
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float acc16_t __attribute__((ext_vector_type(16)));
 
@@ -215,7 +216,8 @@ __global__ __launch_bounds__(64 * kInferWaves, 3) void rc_infer_kernel(RcInferAr
     __shared__ RcLdsFwd W;
     load_fwd_weights(W, a.p.mlp_h);
     __syncthreads();
-    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    if (count > a.stride) count = a.stride;  // appends beyond the queue were dropped
     const int lane = threadIdx.x & 63, n = lane & 31, hi = lane >> 5;
     const uint32_t tiles = (count + 31u) / 32u;
     for (uint32_t tile = blockIdx.x * kInferWaves + threadIdx.x / 64u; tile < tiles; tile += gridDim.x * kInferWaves) {
@@ -240,6 +242,12 @@ __global__ __launch_bounds__(64 * kInferWaves, 3) void rc_infer_kernel(RcInferAr
             }
         }
     }
+}
+
+// Clamp of a scaled table-gradient contribution; NaN passes through, so the
+// pair it lands in is dropped by Adam.
+__device__ inline float clamp_grad(float x) {
+    return x > kRcTableGradClamp ? kRcTableGradClamp : (x < -kRcTableGradClamp ? -kRcTableGradClamp : x);
 }
 
 // Orders a wave's LDS writes before its following reads (and the reads of
@@ -367,16 +375,20 @@ __global__ __launch_bounds__(64 * kTrainWaves) void rc_train_kernel(RcParams p, 
 #pragma unroll
                 for (int jj = 0; jj < 2; jj++) {
                     const int level = 4 * c + 2 * hi + jj;
-                    const float g0 = d[4 * c + 2 * jj] * scale, g1 = d[4 * c + 2 * jj + 1] * scale;
+                    const float gs = scale * kRcTableGradScale;
+                    const float g0 = d[4 * c + 2 * jj] * gs, g1 = d[4 * c + 2 * jj + 1] * gs;
                     if (g0 == 0.0f && g1 == 0.0f) continue;
                     const RcCell cell = rc_locate(p.grid, level, pos);
-                    float* gt = g.table + (((size_t)level << p.grid.log2_table) << 1);
+                    uint32_t* gt = g.table + ((size_t)level << p.grid.log2_table);
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         uint32_t idx;
                         const float w = rc_corner(p.grid, level, cell, k, idx);
-                        atomicAdd(gt + 2 * (size_t)idx, w * g0);
-                        atomicAdd(gt + 2 * (size_t)idx + 1, w * g1);
+                        half2_t v;
+                        v[0] = (_Float16)clamp_grad(w * g0);
+                        v[1] = (_Float16)clamp_grad(w * g1);
+                        typedef __attribute__((address_space(1))) half2_t global_half2;
+                        __builtin_amdgcn_global_atomic_fadd_v2f16((global_half2*)(gt + idx), v);
                     }
                 }
             }
@@ -435,13 +447,11 @@ __global__ void rc_reduce_kernel(const float* partial, uint32_t waves, float* gr
     atomicAdd(grad + i, s);
 }
 
-__device__ inline float adam_update(const RcAdam& a, size_t i) {
-    const float g = a.grad[i] * a.grad_scale;
+__device__ inline float adam_step(const RcAdam& a, size_t i, float g) {
     const float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * g;
     const float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * (g * g);
     a.m[i] = m;
     a.v[i] = v;
-    a.grad[i] = 0.0f;
     const float mh = m / a.bias1;
     const float vh = v / a.bias2;
     const float p = a.master[i] - a.lr * mh / (sqrt_(vh) + a.eps);
@@ -452,7 +462,9 @@ __device__ inline float adam_update(const RcAdam& a, size_t i) {
 __global__ void rc_adam_mlp_kernel(RcAdam a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const _Float16 h = (_Float16)adam_update(a, i);
+    const float g = a.grad[i] * a.grad_scale;
+    a.grad[i] = 0.0f;
+    const _Float16 h = (_Float16)adam_step(a, i, g);
     uint16_t b;
     __builtin_memcpy(&b, &h, 2);
     a.half_mlp[i] = b;
@@ -461,15 +473,19 @@ __global__ void rc_adam_mlp_kernel(RcAdam a) {
 // Table entries: two features per 32-bit FP16 pair.  Entries no sample of
 // the step touched keep their parameters and moments (lazy Adam: the moments
 // of a sparse row decay only when the row is used), so a step costs a read
-// of the gradient plus the touched entries.
+// of the gradient plus the touched entries.  A pair whose scaled sum
+// overflowed is dropped for this step.
 __global__ void rc_adam_table_kernel(RcAdam a) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    const float2* g2 = reinterpret_cast<const float2*>(a.grad);
+    const float unscale = a.grad_scale / kRcTableGradScale;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.n / 2u; e += stride) {
-        const float2 g = g2[e];
-        if (g.x == 0.0f && g.y == 0.0f) continue;
-        const _Float16 h0 = (_Float16)adam_update(a, 2 * (size_t)e);
-        const _Float16 h1 = (_Float16)adam_update(a, 2 * (size_t)e + 1);
+        const uint32_t word = a.grad_h2[e];
+        if ((word & 0x7fff7fffu) == 0u) continue;
+        a.grad_h2[e] = 0u;
+        const float g0 = half_lo(word) * unscale, g1 = half_hi(word) * unscale;
+        if (!isfinite_(g0) || !isfinite_(g1)) continue;
+        const _Float16 h0 = (_Float16)adam_step(a, 2 * (size_t)e, g0);
+        const _Float16 h1 = (_Float16)adam_step(a, 2 * (size_t)e + 1, g1);
         uint16_t b0, b1;
         __builtin_memcpy(&b0, &h0, 2);
         __builtin_memcpy(&b1, &h1, 2);

@@ -8,16 +8,24 @@
 
 namespace rm {
 
-// Parameters on the device.  Masters, moments and gradients are FP32; the
-// kernels read the FP16 copies Adam writes.
+// Parameters on the device.  Masters, moments and MLP gradients are FP32;
+// the kernels read the FP16 copies Adam writes.  Table gradients accumulate
+// as FP16 pairs (one packed atomic per grid corner) scaled by
+// kRcTableGradScale, which keeps per-sample contributions of a mean loss
+// over thousands of samples in FP16's normal range; a contribution is
+// clamped to +-kRcTableGradClamp and Adam skips entries whose sum is not
+// finite.
 struct RcParams {
     RcGrid grid;
     const uint32_t* table_h;   // levels x 2^log2 entries of two FP16 features
     const uint16_t* mlp_h;     // kRcMlpParams FP16 bit patterns, [in][out] per layer
 };
 
+constexpr float kRcTableGradScale = 4096.0f;
+constexpr float kRcTableGradClamp = 2048.0f;
+
 struct RcGrads {
-    float* table;              // levels x 2^log2 x 2
+    uint32_t* table;           // levels x 2^log2 FP16 pairs, x kRcTableGradScale
     float* mlp_partial;        // [waves][kRcMlpParams]
     float* loss;               // sum of per-sample losses (one float)
 };
@@ -46,7 +54,8 @@ struct RcAdam {
     float* master;
     float* m;
     float* v;
-    float* grad;
+    float* grad;               // MLP: FP32 gradients
+    uint32_t* grad_h2;         // table: scaled FP16 gradient pairs
     uint32_t* half_out;        // FP16 copy; two values per word when pairs != 0
     uint16_t* half_mlp;        // FP16 bit patterns
     uint32_t n;
