@@ -232,8 +232,8 @@ def main():
                          "batch: 64 uneven Cornell-box jobs in one bounce loop")
     ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
-    ap.add_argument("--cache-batch", type=int, default=16384, help="cache workload: samples per training step")
-    ap.add_argument("--cache-steps", type=int, default=4, help="cache workload: training steps per sample")
+    ap.add_argument("--cache-batch", type=int, default=8192, help="cache workload: samples per training step")
+    ap.add_argument("--cache-steps", type=int, default=2, help="cache workload: training steps per sample")
     ap.add_argument("--material-sort", type=int, default=-1,
                     help="group hits by material before shading (-1 = on for graphs, off for spheres)")
     ap.add_argument("--inline-materials", type=int, default=-1,

@@ -78,11 +78,12 @@ RM_HD void rc_vertex_features(vec3 wo, vec3 n, float roughness, vec3 albedo, flo
 }
 
 // Queries emitted by the shading kernel: the path ends at this vertex and
-// `throughput x cache(features)` is added to `pixel`.
+// `throughput x cache(features)` is added to `pixel`.  One 64-byte row per
+// query: [0] pixel (bits), [1..3] throughput, [4..15] vertex features, so a
+// lookup read in any order touches one cache line.
+constexpr int kRcQueryFloats = 16;
 struct RcQueryQueue {
-    uint32_t* pixel;
-    float *tx, *ty, *tz;
-    float* feat;  // kRcFeatureFloats x capacity, feature-major
+    float* rows;  // kRcQueryFloats x capacity, row-major
 };
 
 // Vertices of training paths: features, the path throughput on arrival and

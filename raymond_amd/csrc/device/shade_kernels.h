@@ -321,14 +321,21 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
                                reinterpret_cast<uint32_t(&)[kStreams]>(at));
         const uint32_t sslot = at[0], nslot = at[1];
         if (CACHED && out.cache_kind == 1u && at[2] < a.cache.query_capacity) {
-            const RcQueryQueue& q = a.cache.queries;
-            const uint32_t j = at[2];
-            q.pixel[j] = out.pixel;
-            q.tx[j] = out.c_t.x;
-            q.ty[j] = out.c_t.y;
-            q.tz[j] = out.c_t.z;
+            typedef float row4 __attribute__((ext_vector_type(4)));
+            row4* row = reinterpret_cast<row4*>(a.cache.queries.rows + (size_t)at[2] * kRcQueryFloats);
+            row4 r0;
+            r0[0] = __builtin_bit_cast(float, out.pixel);
+            r0[1] = out.c_t.x;
+            r0[2] = out.c_t.y;
+            r0[3] = out.c_t.z;
+            row[0] = r0;
 #pragma unroll
-            for (int f = 0; f < kRcFeatureFloats; f++) q.feat[(size_t)f * a.cache.query_capacity + j] = out.feat[f];
+            for (int c = 0; c < 3; c++) {
+                row4 v;
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = out.feat[4 * c + k];
+                row[1 + c] = v;
+            }
         } else if (CACHED && out.cache_kind == 2u && at[3] < a.cache.train_capacity) {
             const RcTrainQueue& q = a.cache.train;
             const uint32_t j = at[3];

@@ -150,7 +150,25 @@ struct rm_renderer {
     size_t slot_capacity = 0;          // training paths per stream
     size_t query_capacity = 0;         // lookups per stream
     uint32_t cache_frames = 0;         // samples rendered since the cache was attached
-    rm::DevBuf slot_buf, slot_rad_buf, query_buf, train_buf, target_buf, shadow_slot_buf;
+    uint32_t cache_parity = 0;         // which of the two queue sets the next stream fills
+    rm::DevBuf slot_buf, shadow_slot_buf, target_buf;
+    rm::DevBuf slot_rad_buf[2], query_buf[2], train_buf[2];  // per stream parity
+    rm::DevBuf qsort_buf;  // lookup sort: keys, order, predictions, bin starts and cursors
+    // Training and lookups of one stream run on their own HIP stream while
+    // the next stream traces; done[p] marks the end of the cache work that
+    // reads the queue set of parity p.
+    struct CacheOverlap {
+        hipStream_t stream = nullptr;
+        hipEvent_t loop_end = nullptr;
+        hipEvent_t done[2] = {nullptr, nullptr};
+        bool pending[2] = {false, false};
+        ~CacheOverlap() {
+            if (stream) (void)hipStreamDestroy(stream);
+            if (loop_end) (void)hipEventDestroy(loop_end);
+            for (hipEvent_t e : done)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } overlap;
     rm::CacheView cv{};
     rm::vec3 world_lo, world_hi;
     rm::BatchRun* batch = nullptr;     // set while a batch renders through this renderer

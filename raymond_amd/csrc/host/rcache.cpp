@@ -116,8 +116,7 @@ void cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache* c) {
 
     hipStream_t s = dev->stream;
     c->t_master.alloc(c->table_floats * 4);
-    c->t_m.alloc(c->table_floats * 4);
-    c->t_v.alloc(c->table_floats * 4);
+    c->t_state.alloc(c->table_floats * 4 * 4);
     c->t_grad.alloc(c->table_floats * 2);
     c->t_half.alloc(c->table_floats * 2);
     c->w_master.alloc(kRcMlpParams * 4);
@@ -126,8 +125,7 @@ void cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache* c) {
     c->w_grad.alloc(kRcMlpParams * 4);
     c->w_half.alloc(kRcMlpParams * 2);
     c->loss.alloc(4);
-    RM_HIP(launch_rc_fill(c->t_m.as<float>(), 0.0f, c->table_floats, s));
-    RM_HIP(launch_rc_fill(c->t_v.as<float>(), 0.0f, c->table_floats, s));
+    RM_HIP(launch_rc_fill(c->t_state.as<float>(), 0.0f, c->table_floats * 4, s));
     RM_HIP(launch_rc_fill(c->w_m.as<float>(), 0.0f, kRcMlpParams, s));
     RM_HIP(launch_rc_fill(c->w_v.as<float>(), 0.0f, kRcMlpParams, s));
     zero_grads(c, s);
@@ -142,8 +140,8 @@ RcParams cache_params(const rm_cache* c) {
     return p;
 }
 
-float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_loss) {
-    hipStream_t s = c->dev->stream;
+float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_loss, hipStream_t stream) {
+    hipStream_t s = stream ? stream : c->dev->stream;
     RM_HIP(launch_rc_fill(c->loss.as<float>(), 0.0f, 1, s));
     cache_backward(c, smp, batch, s);
     c->steps++;
@@ -163,9 +161,8 @@ float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_lo
     a.half_mlp = c->w_half.as<uint16_t>();
     a.n = kRcMlpParams;
     RM_HIP(launch_rc_adam_mlp(a, s));
-    a.master = c->t_master.as<float>();
-    a.m = c->t_m.as<float>();
-    a.v = c->t_v.as<float>();
+    a.master = a.m = a.v = nullptr;
+    a.state = c->t_state.as<float>();
     a.grad = nullptr;
     a.grad_h2 = c->t_grad.as<uint32_t>();
     a.half_out = c->t_half.as<uint32_t>();
@@ -187,6 +184,8 @@ void cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table, 
     }
     if (table) {
         RM_REQUIRE(table_count >= c->table_floats, "table buffer too small");
+        RM_HIP(launch_rc_table_master(c->t_state.as<float>(), c->t_master.as<float>(), c->t_half.as<uint32_t>(),
+                                      c->table_floats, false, s));
         RM_HIP(hipMemcpyAsync(table, c->t_master.ptr, c->table_floats * 4, hipMemcpyDeviceToHost, s));
     }
     RM_HIP(hipStreamSynchronize(s));
@@ -201,9 +200,10 @@ void cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const f
     if (table) {
         RM_REQUIRE(table_count >= c->table_floats, "table buffer too small");
         RM_HIP(hipMemcpyAsync(c->t_master.ptr, table, c->table_floats * 4, hipMemcpyHostToDevice, s));
+        RM_HIP(launch_rc_table_master(c->t_state.as<float>(), c->t_master.as<float>(), c->t_half.as<uint32_t>(),
+                                      c->table_floats, true, s));
     }
-    RM_HIP(launch_rc_halves(c->w_master.as<float>(), c->w_half.as<uint16_t>(), c->t_master.as<float>(),
-                            c->t_half.as<uint32_t>(), c->table_floats, s));
+    RM_HIP(launch_rc_halves(c->w_master.as<float>(), c->w_half.as<uint16_t>(), s));
     RM_HIP(hipStreamSynchronize(s));
 }
 
@@ -215,7 +215,7 @@ void cache_infer_raw(rm_cache* c, const float* features, uint32_t n, float* out)
     c->stage_out.alloc((size_t)n * 3 * 4);
     RcInferArgs a{};
     a.p = cache_params(c);
-    a.q.feat = c->stage_feat.as<float>();
+    a.feat = c->stage_feat.as<float>();
     a.count = n;
     a.stride = n;
     a.out = c->stage_out.as<float>();

@@ -9,7 +9,9 @@ struct rm_cache {
     rm::RcGrid grid{};
     size_t table_floats = 0;
     // FP32 masters, Adam moments, gradients and the FP16 copies the kernels read
-    rm::DevBuf t_master, t_m, t_v, t_grad, t_half;
+    rm::DevBuf t_state;            // Adam records, 8 floats per table pair
+    rm::DevBuf t_master;           // FP32 table staging for parameter reads and writes
+    rm::DevBuf t_grad, t_half;
     rm::DevBuf w_master, w_m, w_v, w_grad, w_half;
     rm::DevBuf partial, loss;
     uint32_t partial_waves = 0;
@@ -26,7 +28,8 @@ void cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache* c);
 RcParams cache_params(const rm_cache* c);
 // One Adam step on `batch` samples drawn from `smp` (its permutation fields
 // select them); returns the mean loss only when `read_loss` (it synchronises).
-float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_loss);
+// One Adam step on `batch` samples, on `stream` (null: the device's stream).
+float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_loss, hipStream_t stream = nullptr);
 void cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table, size_t table_count);
 void cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const float* table, size_t table_count);
 void cache_infer_raw(rm_cache* c, const float* features, uint32_t n, float* out);

@@ -143,25 +143,38 @@ void renderer_set_cache(rm_renderer* r, rm_cache* c, uint32_t mode) {
     r->world_hi = hi;
 }
 
-// Buffers of the radiance cache for a stream of `samples` samples per pixel.
-static void prepare_cache(rm_renderer* r, uint32_t samples) {
+// Waits for the cache work of both queue sets (before buffers grow, and
+// before a render call returns).
+static void join_cache_work(rm_renderer* r) {
+    if (r->overlap.stream) RM_HIP(hipStreamSynchronize(r->overlap.stream));
+    r->overlap.pending[0] = r->overlap.pending[1] = false;
+}
+
+// Buffers of the radiance cache for a stream of `samples` samples per pixel,
+// filling the queue set of parity `par`.  They are sized for the longest
+// stream at once, so that growing streams allocate nothing on the way.
+static void prepare_cache(rm_renderer* r, uint32_t samples, uint32_t par) {
     rm_cache* c = r->cache;
     CacheView& cv = r->cv;
     const size_t cap = r->capacity;
     const bool train = r->cache_mode == 1u;
     const uint32_t shift = c->desc.train_shift;
-    const size_t total = (size_t)r->width * r->height * samples;
+    const size_t total = (size_t)r->width * r->height * std::max(samples, kCacheStreamMax);
     const size_t slots = train ? ((total + (1ull << shift) - 1) >> shift) + 1 : 0;
     // every path of the stream looks up at most once
-    const size_t queries = std::max<size_t>(cap, (size_t)r->width * r->height * samples);
+    const size_t queries = std::max<size_t>(cap, total);
+    if (queries > r->query_capacity || (train && slots > r->slot_capacity)) join_cache_work(r);
     if (queries > r->query_capacity) {
         r->query_capacity = queries;
-        r->query_buf.alloc(queries * (4 + kRcFeatureFloats) * 4);
+        for (DevBuf& b : r->query_buf) b.alloc(queries * kRcQueryFloats * 4);
+        // keys, order (u32 each), predictions (4 floats), bin starts + cursors
+        r->qsort_buf.alloc(queries * 6 * 4 + 2 * kRcSortBins * 4);
+        RM_HIP(hipMemsetAsync(r->qsort_buf.ptr, 0, r->qsort_buf.bytes, r->scene->dev->stream));
     }
     if (train && slots > r->slot_capacity) {
         r->slot_capacity = slots;
-        r->slot_rad_buf.alloc(slots * 3 * 4);
-        r->train_buf.alloc(slots * (7 + kRcFeatureFloats) * 4);
+        for (DevBuf& b : r->slot_rad_buf) b.alloc(slots * 3 * 4);
+        for (DevBuf& b : r->train_buf) b.alloc(slots * (7 + kRcFeatureFloats) * 4);
         r->target_buf.alloc(slots * 3 * 4);
     }
     if (train && !r->slot_buf.ptr) {
@@ -177,18 +190,11 @@ static void prepare_cache(rm_renderer* r, uint32_t samples) {
     cv.bounds_min = r->world_lo;
     const vec3 ext = r->world_hi - r->world_lo;
     cv.bounds_inv_extent = vec3(1.0f / max_(ext.x, 1e-6f), 1.0f / max_(ext.y, 1e-6f), 1.0f / max_(ext.z, 1e-6f));
-    const size_t qc = r->query_capacity;
-    uint32_t* qu = r->query_buf.as<uint32_t>();
-    float* qf = r->query_buf.as<float>();
-    cv.queries.pixel = qu;
-    cv.queries.tx = qf + qc;
-    cv.queries.ty = qf + 2 * qc;
-    cv.queries.tz = qf + 3 * qc;
-    cv.queries.feat = qf + 4 * qc;
+    cv.queries.rows = r->query_buf[par].as<float>();
     if (train) {
         const size_t sc = r->slot_capacity;
-        float* tf = r->train_buf.as<float>();
-        cv.train.slot = r->train_buf.as<uint32_t>();
+        float* tf = r->train_buf[par].as<float>();
+        cv.train.slot = r->train_buf[par].as<uint32_t>();
         cv.train.tx = tf + sc;
         cv.train.ty = tf + 2 * sc;
         cv.train.tz = tf + 3 * sc;
@@ -196,8 +202,16 @@ static void prepare_cache(rm_renderer* r, uint32_t samples) {
         cv.train.ry = tf + 5 * sc;
         cv.train.rz = tf + 6 * sc;
         cv.train.feat = tf + 7 * sc;
-        cv.slot_radiance = r->slot_rad_buf.as<float>();
+        cv.slot_radiance = r->slot_rad_buf[par].as<float>();
     }
+}
+
+static void CacheOverlapInit(rm_renderer* r) {
+    auto& o = r->overlap;
+    if (o.stream) return;
+    RM_HIP(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
+    RM_HIP(hipEventCreateWithFlags(&o.loop_end, hipEventDisableTiming));
+    for (hipEvent_t& e : o.done) RM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 // Records the completion time of every job whose paths have all ended.
@@ -285,7 +299,14 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
 
     const bool cached = r->cache_mode != 0u;
     const bool cache_train = r->cache_mode == 1u;
-    if (cached) prepare_cache(r, samples);
+    // the cache work overlaps the next stream unless stages are being timed
+    // or the stream's image is accumulated right after it
+    const bool overlap = cached && !tm.on && !r->u.accumulate_f64;
+    const uint32_t par = r->cache_parity;
+    if (cached) {
+        prepare_cache(r, samples, par);
+        if (r->overlap.pending[par]) RM_HIP(hipStreamWaitEvent(st, r->overlap.done[par], 0));
+    }
     uint32_t* slot_arr[2] = {nullptr, nullptr};
     if (cache_train) {
         slot_arr[0] = r->slot_buf.as<uint32_t>();
@@ -420,6 +441,13 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         RM_HIP(hipStreamSynchronize(st));
         rm_cache* c = r->cache;
         c->queries += cache_ctr[0];
+        hipStream_t cs = st;
+        if (overlap) {
+            CacheOverlapInit(r);
+            cs = r->overlap.stream;
+            RM_HIP(hipEventRecord(r->overlap.loop_end, st));
+            RM_HIP(hipStreamWaitEvent(cs, r->overlap.loop_end, 0));
+        }
         const uint32_t records = std::min<uint32_t>(cache_ctr[1], r->cv.train_capacity);
         if (cache_train && records > 0) {
             // targets of this stream's training vertices, then Adam steps on
@@ -427,19 +455,35 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             // stream's lookups so that they see what the stream taught
             std::vector<std::pair<hipEvent_t, hipEvent_t>> t_train;
             hipEvent_t et = tm.begin();
-            RM_HIP(launch_rc_targets(r->cv.train, r->cv.slot_radiance, &cnt->train_records, r->cv.train_capacity,
-                                     r->target_buf.as<float>(), st));
+            RM_HIP(launch_rc_targets(r->cv.train, r->cv.slot_radiance, records, r->cv.train_capacity,
+                                     r->target_buf.as<float>(), cs));
+            // the records in spatial order (the lookup sort's buffers are
+            // free until the lookups), sampled systematically per step
+            uint32_t* qs = r->qsort_buf.as<uint32_t>();
+            const size_t qc = r->query_capacity;
+            RcSortArgs so{};
+            so.pos = r->cv.train.feat;
+            so.pos_elem = 1;
+            so.pos_comp = r->cv.train_capacity;
+            so.count = records;
+            so.keys = qs;
+            so.order = qs + qc;
+            so.hist = qs + 6 * qc;
+            so.cursor = so.hist + kRcSortBins;
+            RM_HIP(launch_rc_sort_queries(so, cus * 4, cs));
             RcSamples smp{};
             smp.feat = r->cv.train.feat;
             smp.target = r->target_buf.as<float>();
             smp.stride = r->cv.train_capacity;
             smp.count = records;
-            smp.perm_mul = 2147483647u;  // prime, so a permutation of any smaller count
+            smp.order = so.order;
             const uint32_t batch = std::min<uint32_t>(c->desc.batch_size, records);
             const uint32_t steps = c->desc.steps_per_launch * samples;
             for (uint32_t k = 0; k < steps; k++) {
-                smp.first = (uint32_t)(c->steps * 7919u) + k * batch;
-                cache_step(c, smp, batch, false);
+                // golden-ratio sequence of offsets
+                const double u = (double)(c->steps + 1) * 0.6180339887498949;
+                smp.offset = (float)(u - std::floor(u));
+                cache_step(c, smp, batch, false, cs);
             }
             tm.end(et, t_train);
             c->records += records;
@@ -448,17 +492,41 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
                 c->ms_train += StageTimer::sum(t_train);
             }
         }
-        // the lookups of the whole stream in one batch
-        RcInferArgs ia{};
-        ia.p = cache_params(r->cache);
-        ia.q = r->cv.queries;
-        ia.count_ptr = &cnt->cache_queries;
-        ia.stride = r->cv.query_capacity;
-        ia.image = image;
+        // the lookups of the whole stream in one batch (counts by value: the
+        // next stream clears the counters): sorted by position, predicted,
+        // then added to the image in queue order
+        const uint32_t nq = std::min<uint32_t>(cache_ctr[0], r->cv.query_capacity);
         hipEvent_t ei = tm.begin();
-        RM_HIP(launch_rc_infer(ia, cus * 8, st));
+        if (nq > 0) {
+            const size_t qc = r->query_capacity;
+            uint32_t* qs = r->qsort_buf.as<uint32_t>();
+            RcSortArgs so{};
+            so.pos = r->cv.queries.rows + 4;
+            so.pos_elem = kRcQueryFloats;
+            so.pos_comp = 1;
+            so.count = nq;
+            so.keys = qs;
+            so.order = qs + qc;
+            float* pred = reinterpret_cast<float*>(qs + 2 * qc);
+            so.hist = qs + 6 * qc;
+            so.cursor = so.hist + kRcSortBins;
+            RM_HIP(launch_rc_sort_queries(so, cus * 4, cs));
+            RcInferArgs ia{};
+            ia.p = cache_params(r->cache);
+            ia.rows = r->cv.queries.rows;
+            ia.order = so.order;
+            ia.count = nq;
+            ia.out = pred;
+            RM_HIP(launch_rc_infer(ia, cus * 8, cs));
+            RM_HIP(launch_rc_resolve(ia.rows, pred, nq, image, cs));
+        }
         tm.end(ei, t_infer);
+        if (overlap) {
+            RM_HIP(hipEventRecord(r->overlap.done[par], cs));
+            r->overlap.pending[par] = true;
+        }
         r->cache_frames += samples;
+        r->cache_parity = par ^ 1u;
     }
     if (r->u.accumulate_f64) {
         if (!r->image64_buf.ptr) {
@@ -531,9 +599,15 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
             while (max_stream * 2 <= r->cache_frames && max_stream * 2 <= kCacheStreamMax) max_stream *= 2;
         }
         const uint32_t n = left < max_stream ? left : max_stream;
-        render_stream(r, n, stats);
+        try {
+            render_stream(r, n, stats);
+        } catch (...) {
+            join_cache_work(r);
+            throw;
+        }
         left -= n;
     }
+    join_cache_work(r);
     if (stats) stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

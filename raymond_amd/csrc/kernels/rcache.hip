@@ -203,44 +203,163 @@ __device__ inline void rc_forward(const RcLdsFwd& W, const half4_t x[6], int n, 
     act.y = y;
 }
 
-__device__ inline void rc_image_add(float* image, uint32_t pixel, float r, float g, float b) {
-    float* px = image + 4u * (size_t)pixel;
-    atomicAdd(px + 0, r);
-    atomicAdd(px + 1, g);
-    atomicAdd(px + 2, b);
-    atomicAdd(px + 3, 1.0f);
-}
-
 // Three waves per SIMD is the most the register budget allows without spills.
+// Query rows are read through the spatial order; when the grid is a multiple
+// of 8 workgroups, the tiles are cut into 8 contiguous ranges, one per
+// workgroup residue mod 8 (consecutive workgroups go to different XCDs), so
+// that each XCD's L2 serves one region of the scene.
 __global__ __launch_bounds__(64 * kInferWaves, 3) void rc_infer_kernel(RcInferArgs a) {
     __shared__ RcLdsFwd W;
     load_fwd_weights(W, a.p.mlp_h);
     __syncthreads();
-    uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
-    if (count > a.stride) count = a.stride;  // appends beyond the queue were dropped
+    typedef float row4 __attribute__((ext_vector_type(4)));
+    const uint32_t count = a.count;
     const int lane = threadIdx.x & 63, n = lane & 31, hi = lane >> 5;
     const uint32_t tiles = (count + 31u) / 32u;
-    for (uint32_t tile = blockIdx.x * kInferWaves + threadIdx.x / 64u; tile < tiles; tile += gridDim.x * kInferWaves) {
+    const uint32_t wave = threadIdx.x / 64u;
+    uint32_t t1 = tiles, first = blockIdx.x * kInferWaves + wave, step = gridDim.x * kInferWaves;
+    if (a.rows && (gridDim.x & 7u) == 0u) {
+        const uint32_t part = blockIdx.x & 7u, per = (tiles + 7u) / 8u;
+        const uint32_t t0 = part * per;
+        t1 = min(tiles, t0 + per);
+        first = t0 + (blockIdx.x >> 3) * kInferWaves + wave;
+        step = (gridDim.x >> 3) * kInferWaves;
+    }
+    for (uint32_t tile = first; tile < t1; tile += step) {
         const uint32_t i = tile * 32u + (uint32_t)n;
         const bool valid = i < count;
         float f[kRcFeatureFloats];
+        uint32_t row = i;
+        if (a.rows) {
+            row = valid ? (a.order ? a.order[i] : i) : 0u;
+            const row4* r = reinterpret_cast<const row4*>(a.rows + (size_t)row * kRcQueryFloats);
 #pragma unroll
-        for (int k = 0; k < kRcFeatureFloats; k++) f[k] = valid ? a.q.feat[(size_t)k * a.stride + i] : 0.0f;
+            for (int c = 0; c < 3; c++) {
+                const row4 v = valid ? r[1 + c] : row4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int k = 0; k < 4; k++) f[4 * c + k] = v[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRcFeatureFloats; k++) f[k] = valid ? a.feat[(size_t)k * a.stride + i] : 0.0f;
+        }
         half4_t x[6];
         rc_inputs(a.p, f, hi, x);
         RcAct act;
         rc_forward(W, x, n, hi, act);
         if (hi == 0 && valid) {
-            if (a.image) {
-                const float tx = a.q.tx[i], ty = a.q.ty[i], tz = a.q.tz[i];
-                rc_image_add(a.image, a.q.pixel[i], tx * max_(act.y[0], 0.0f), ty * max_(act.y[1], 0.0f),
-                             tz * max_(act.y[2], 0.0f));
+            if (a.rows) {
+                row4 v;
+                v[0] = max_(act.y[0], 0.0f);
+                v[1] = max_(act.y[1], 0.0f);
+                v[2] = max_(act.y[2], 0.0f);
+                v[3] = 0.0f;
+                reinterpret_cast<row4*>(a.out)[row] = v;
             } else {
                 a.out[i] = act.y[0];
                 a.out[(size_t)count + i] = act.y[1];
                 a.out[2 * (size_t)count + i] = act.y[2];
             }
         }
+    }
+}
+
+// Morton code of a position in [0, 1]^3 on the sort grid.
+__device__ inline uint32_t rc_sort_key(const RcSortArgs& s, uint32_t i) {
+    constexpr float cells = (float)(1u << kRcSortBits);
+    uint32_t key = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float v = s.pos[(size_t)i * s.pos_elem + (size_t)a * s.pos_comp] * cells;
+        uint32_t c = v > 0.0f ? (uint32_t)v : 0u;
+        c = c < (1u << kRcSortBits) ? c : (1u << kRcSortBits) - 1u;
+#pragma unroll
+        for (uint32_t b = 0; b < kRcSortBits; b++) key |= ((c >> b) & 1u) << (3u * b + (uint32_t)a);
+    }
+    return key;
+}
+
+constexpr int kRcSortBlock = 256;
+
+__global__ __launch_bounds__(kRcSortBlock) void rc_qkeys_kernel(RcSortArgs a) {
+    __shared__ uint32_t hist[kRcSortBins];
+    for (uint32_t b = threadIdx.x; b < kRcSortBins; b += kRcSortBlock) hist[b] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kRcSortBlock + threadIdx.x; i < a.count; i += gridDim.x * kRcSortBlock) {
+        const uint32_t k = rc_sort_key(a, i);
+        a.keys[i] = k;
+        atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kRcSortBins; b += kRcSortBlock)
+        if (hist[b]) atomicAdd(&a.hist[b], hist[b]);
+}
+
+// One workgroup: exclusive prefix of the bin counts, in place.
+__global__ __launch_bounds__(1024) void rc_qscan_kernel(RcSortArgs a) {
+    constexpr uint32_t per = kRcSortBins / 1024u;
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    uint32_t v[per], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        v[k] = a.hist[t * per + k];
+        sum += v[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t add = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        a.hist[t * per + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ __launch_bounds__(kRcSortBlock) void rc_qscatter_kernel(RcSortArgs a) {
+    __shared__ uint32_t cnt[kRcSortBins];
+    __shared__ uint32_t base[kRcSortBins];
+    for (uint32_t b = threadIdx.x; b < kRcSortBins; b += kRcSortBlock) cnt[b] = 0u;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * kRcSortBlock;
+    for (uint32_t i = blockIdx.x * kRcSortBlock + threadIdx.x; i < a.count; i += stride) atomicAdd(&cnt[a.keys[i]], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kRcSortBins; b += kRcSortBlock) {
+        const uint32_t c = cnt[b];
+        base[b] = c ? a.hist[b] + atomicAdd(&a.cursor[b], c) : 0u;
+        cnt[b] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kRcSortBlock + threadIdx.x; i < a.count; i += stride) {
+        const uint32_t k = a.keys[i];
+        a.order[base[k] + atomicAdd(&cnt[k], 1u)] = i;
+    }
+}
+
+// Clears the starts and cursors after the scatter, for the next sort.
+__global__ void rc_qclear_kernel(RcSortArgs a) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < kRcSortBins; b += gridDim.x * blockDim.x) {
+        a.hist[b] = 0u;
+        a.cursor[b] = 0u;
+    }
+}
+
+__global__ void rc_resolve_kernel(const float* rows, const float* pred, uint32_t count, float* image) {
+    typedef float row4 __attribute__((ext_vector_type(4)));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const row4 r = reinterpret_cast<const row4*>(rows)[(size_t)i * (kRcQueryFloats / 4)];
+        const row4 p = reinterpret_cast<const row4*>(pred)[i];
+        float* px = image + 4u * (size_t)__builtin_bit_cast(uint32_t, r[0]);
+        atomicAdd(px + 0, r[1] * p[0]);
+        atomicAdd(px + 1, r[2] * p[1]);
+        atomicAdd(px + 2, r[3] * p[2]);
+        atomicAdd(px + 3, 1.0f);
     }
 }
 
@@ -301,7 +420,14 @@ __global__ __launch_bounds__(64 * kTrainWaves) void rc_train_kernel(RcParams p, 
     const uint32_t j = gwave * 32u + (uint32_t)n;
     const bool valid = j < batch && smp.count > 0;
     uint32_t rec = 0;
-    if (valid) rec = (uint32_t)((((uint64_t)smp.first + j) * smp.perm_mul) % smp.count);
+    if (valid) {
+        if (smp.order) {
+            const uint32_t q = (uint32_t)(((double)j + (double)smp.offset) * (double)smp.count / (double)batch);
+            rec = smp.order[q < smp.count ? q : smp.count - 1u];
+        } else {
+            rec = (uint32_t)((((uint64_t)smp.first + j) * smp.perm_mul) % smp.count);
+        }
+    }
     float f[kRcFeatureFloats];
 #pragma unroll
     for (int k = 0; k < kRcFeatureFloats; k++) f[k] = valid ? smp.feat[(size_t)k * smp.stride + rec] : 0.0f;
@@ -470,32 +596,67 @@ __global__ void rc_adam_mlp_kernel(RcAdam a) {
     a.half_mlp[i] = b;
 }
 
-// Table entries: two features per 32-bit FP16 pair.  Entries no sample of
-// the step touched keep their parameters and moments (lazy Adam: the moments
-// of a sparse row decay only when the row is used), so a step costs a read
-// of the gradient plus the touched entries.  A pair whose scaled sum
-// overflowed is dropped for this step.
+// Table entries: two features per 32-bit FP16 pair, and per pair one
+// 32-byte Adam record {p0, p1, m0, m1, v0, v1, -, -} (one sector per touched
+// entry).  Entries no sample of the step touched keep their parameters and
+// moments (lazy Adam: the moments of a sparse row decay only when the row is
+// used), so a step costs a read of the gradient plus the touched entries.  A
+// pair whose scaled sum overflowed is dropped for this step.
 __global__ void rc_adam_table_kernel(RcAdam a) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const uint32_t stride = gridDim.x * blockDim.x;
     const float unscale = a.grad_scale / kRcTableGradScale;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.n / 2u; e += stride) {
         const uint32_t word = a.grad_h2[e];
         if ((word & 0x7fff7fffu) == 0u) continue;
         a.grad_h2[e] = 0u;
-        const float g0 = half_lo(word) * unscale, g1 = half_hi(word) * unscale;
-        if (!isfinite_(g0) || !isfinite_(g1)) continue;
-        const _Float16 h0 = (_Float16)adam_step(a, 2 * (size_t)e, g0);
-        const _Float16 h1 = (_Float16)adam_step(a, 2 * (size_t)e + 1, g1);
-        uint16_t b0, b1;
-        __builtin_memcpy(&b0, &h0, 2);
-        __builtin_memcpy(&b1, &h1, 2);
-        a.half_out[e] = (uint32_t)b0 | ((uint32_t)b1 << 16);
+        const float g[2] = {half_lo(word) * unscale, half_hi(word) * unscale};
+        if (!isfinite_(g[0]) || !isfinite_(g[1])) continue;
+        f4* rec = reinterpret_cast<f4*>(a.state + 8 * (size_t)e);
+        f4 pm = rec[0], vv = rec[1];
+        uint16_t bits[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float m = a.beta1 * pm[2 + k] + (1.0f - a.beta1) * g[k];
+            const float v = a.beta2 * vv[k] + (1.0f - a.beta2) * (g[k] * g[k]);
+            const float mh = m / a.bias1;
+            const float vh = v / a.bias2;
+            const float p = pm[k] - a.lr * mh / (sqrt_(vh) + a.eps);
+            pm[k] = p;
+            pm[2 + k] = m;
+            vv[k] = v;
+            const _Float16 h = (_Float16)p;
+            __builtin_memcpy(&bits[k], &h, 2);
+        }
+        rec[0] = pm;
+        rec[1] = vv;
+        a.half_out[e] = (uint32_t)bits[0] | ((uint32_t)bits[1] << 16);
     }
 }
 
-__global__ void rc_targets_kernel(RcTrainQueue q, const float* slot_radiance, const uint32_t* count_ptr,
-                                  uint32_t capacity, float* target) {
-    const uint32_t c = *count_ptr;
+// Master table values in or out of the Adam records (the moments stay);
+// setting them also writes the FP16 copy.
+__global__ void rc_table_master_kernel(float* state, float* master, uint32_t* table_h, size_t pairs, int set) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < pairs; e += stride) {
+        float* rec = state + 8 * e;
+        if (set) {
+            rec[0] = master[2 * e];
+            rec[1] = master[2 * e + 1];
+            const _Float16 h0 = (_Float16)rec[0], h1 = (_Float16)rec[1];
+            uint16_t b0, b1;
+            __builtin_memcpy(&b0, &h0, 2);
+            __builtin_memcpy(&b1, &h1, 2);
+            table_h[e] = (uint32_t)b0 | ((uint32_t)b1 << 16);
+        } else {
+            master[2 * e] = rec[0];
+            master[2 * e + 1] = rec[1];
+        }
+    }
+}
+
+__global__ void rc_targets_kernel(RcTrainQueue q, const float* slot_radiance, uint32_t c, uint32_t capacity,
+                                  float* target) {
     const uint32_t count = c < capacity ? c : capacity;
     const float nan = __builtin_nanf("");
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
@@ -513,19 +674,10 @@ __global__ void rc_fill_kernel(float* p, float v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void rc_halves_kernel(const float* mlp, uint16_t* mlp_bits, const float* table, uint32_t* table_h,
-                                 size_t pairs) {
+__global__ void rc_halves_kernel(const float* mlp, uint16_t* mlp_bits) {
     _Float16* mlp_h = reinterpret_cast<_Float16*>(mlp_bits);
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    for (size_t i = t0; i < (size_t)kRcMlpParams; i += stride) mlp_h[i] = (_Float16)mlp[i];
-    for (size_t e = t0; e < pairs; e += stride) {
-        const _Float16 h0 = (_Float16)table[2 * e], h1 = (_Float16)table[2 * e + 1];
-        uint16_t b0, b1;
-        __builtin_memcpy(&b0, &h0, 2);
-        __builtin_memcpy(&b1, &h1, 2);
-        table_h[e] = (uint32_t)b0 | ((uint32_t)b1 << 16);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)kRcMlpParams; i += gridDim.x * blockDim.x)
+        mlp_h[i] = (_Float16)mlp[i];
 }
 
 uint32_t rc_train_waves(uint32_t batch) {
@@ -534,6 +686,17 @@ uint32_t rc_train_waves(uint32_t batch) {
 
 hipError_t launch_rc_infer(const RcInferArgs& a, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(rc_infer_kernel, dim3(blocks), dim3(64 * kInferWaves), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_rc_sort_queries(const RcSortArgs& a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(rc_qkeys_kernel, dim3(blocks), dim3(kRcSortBlock), 0, s, a);
+    hipLaunchKernelGGL(rc_qscan_kernel, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(rc_qscatter_kernel, dim3(blocks), dim3(kRcSortBlock), 0, s, a);
+    hipLaunchKernelGGL(rc_qclear_kernel, dim3(16), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_rc_resolve(const float* rows, const float* pred, uint32_t count, float* image, hipStream_t s) {
+    hipLaunchKernelGGL(rc_resolve_kernel, dim3(2048), dim3(256), 0, s, rows, pred, count, image);
     return hipGetLastError();
 }
 hipError_t launch_rc_train(const RcParams& p, const RcSamples& smp, uint32_t batch, const RcGrads& g, float scale,
@@ -555,9 +718,9 @@ hipError_t launch_rc_adam_table(const RcAdam& a, hipStream_t s) {
     hipLaunchKernelGGL(rc_adam_table_kernel, dim3(2048), dim3(256), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_rc_targets(const RcTrainQueue& q, const float* slot_radiance, const uint32_t* count_ptr,
-                             uint32_t capacity, float* target, hipStream_t s) {
-    hipLaunchKernelGGL(rc_targets_kernel, dim3(1024), dim3(256), 0, s, q, slot_radiance, count_ptr, capacity, target);
+hipError_t launch_rc_targets(const RcTrainQueue& q, const float* slot_radiance, uint32_t count, uint32_t capacity,
+                             float* target, hipStream_t s) {
+    hipLaunchKernelGGL(rc_targets_kernel, dim3(1024), dim3(256), 0, s, q, slot_radiance, count, capacity, target);
     return hipGetLastError();
 }
 hipError_t launch_rc_fill(float* p, float v, size_t n, hipStream_t s) {
@@ -568,9 +731,14 @@ hipError_t launch_rc_fill(float* p, float v, size_t n, hipStream_t s) {
 }  // namespace rm
 
 namespace rm {
-hipError_t launch_rc_halves(const float* mlp, uint16_t* mlp_h, const float* table, uint32_t* table_h,
-                            size_t table_floats, hipStream_t s) {
-    hipLaunchKernelGGL(rc_halves_kernel, dim3(2048), dim3(256), 0, s, mlp, mlp_h, table, table_h, table_floats / 2);
+hipError_t launch_rc_halves(const float* mlp, uint16_t* mlp_h, hipStream_t s) {
+    hipLaunchKernelGGL(rc_halves_kernel, dim3(36), dim3(256), 0, s, mlp, mlp_h);
+    return hipGetLastError();
+}
+hipError_t launch_rc_table_master(float* state, float* master, uint32_t* table_h, size_t table_floats, bool set,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(rc_table_master_kernel, dim3(2048), dim3(256), 0, s, state, master, table_h, table_floats / 2,
+                       set ? 1 : 0);
     return hipGetLastError();
 }
 }  // namespace rm
