@@ -282,7 +282,7 @@ typedef struct rm_cache_desc {
     float learning_rate;       /* Adam; default 1e-2 */
     uint32_t train_shift;      /* one camera path in 2^n trains; default 4 */
     uint32_t batch_size;       /* samples per training step; default 16384 */
-    uint32_t steps_per_launch; /* training steps after each bounce loop; default 4 */
+    uint32_t steps_per_launch; /* training steps per sample per pixel rendered; default 4 */
 } rm_cache_desc;
 
 typedef struct rm_cache_stats {
@@ -306,8 +306,9 @@ int rm_cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table
 int rm_cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const float* table, size_t table_count);
 /* Raw batches (host arrays, n x RM_CACHE_FEATURES row-major inputs):
  * infer writes n x 3 outputs; gradients computes the mean-loss gradients
- * without updating (grad_table may be null); train takes `steps` Adam steps
- * on the batch. */
+ * without updating (grad_table may be null; table gradients are summed in
+ * scaled FP16 on the device and returned as FP32); train takes `steps` Adam
+ * steps on the batch. */
 int rm_cache_infer(rm_cache* c, const float* features, uint32_t n, float* out);
 int rm_cache_gradients(rm_cache* c, const float* features, const float* targets, uint32_t n, float* grad_mlp,
                        float* grad_table, float* loss);
