@@ -79,72 +79,78 @@ struct RayState {
         n_tris = 0;
     }
 
-    // One node and the triangles of its leaf children, or one stack pop.
+    // One node and the triangles of its leaf children (after a stack pop
+    // when the current group is used up, so that no step is a bare pop).
     // Returns true when the traversal has ended (ANY: also at the first hit
     // in (tmin, tmax)).
     template <bool ANY, bool STATS>
     __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, const StackRef& stk) {
-        uint2 tri = make_uint2(0u, 0u);
-        if (cur.y & 0xff000000u) {
-            const uint32_t hits = cur.y >> 24;
-            const int k = __builtin_ctz(hits);
-            cur.y &= ~(1u << (24 + k));
-            const int slot = k ^ oct;
-            const uint32_t node_idx = cur.x + __builtin_popcount(cur.y & 0xffu & ((1u << slot) - 1u));
-            if (cur.y & 0xff000000u) {
-                stk.store(sp, cur);
-                sp++;
-            }
-            const uint4* np = reinterpret_cast<const uint4*>(nodes + node_idx);
-            const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
-            if (STATS) n_nodes++;
-            const float px = __uint_as_float(q0.x), py = __uint_as_float(q0.y), pz = __uint_as_float(q0.z);
-            const float sx = exp_scale((uint8_t)(q0.w & 0xffu));
-            const float sy = exp_scale((uint8_t)((q0.w >> 8) & 0xffu));
-            const float sz = exp_scale((uint8_t)((q0.w >> 16) & 0xffu));
-            const uint32_t imask = q0.w >> 24;
-            uint32_t ihits = 0, thits = 0;
-            float near_t = kInf;
-            int near_k = 0;
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const uint32_t meta = byte_of(q1.z, q1.w, c);
-                if (meta == 0) continue;
-                const vec3 lo(decode_q(px, (uint8_t)byte_of(q2.x, q2.y, c), sx),
-                              decode_q(py, (uint8_t)byte_of(q2.z, q2.w, c), sy),
-                              decode_q(pz, (uint8_t)byte_of(q3.x, q3.y, c), sz));
-                const vec3 hi(decode_q(px, (uint8_t)byte_of(q3.z, q3.w, c), sx),
-                              decode_q(py, (uint8_t)byte_of(q4.x, q4.y, c), sy),
-                              decode_q(pz, (uint8_t)byte_of(q4.z, q4.w, c), sz));
-                float tn;
-                if (box_hit(lo, hi, o, inv, tmin, best_t, tn)) {
-                    if ((imask >> c) & 1u) {
-                        ihits |= 1u << (c ^ oct);
-                        if (tn < near_t) {
-                            near_t = tn;
-                            near_k = c ^ oct;
-                        }
-                    } else {
-                        thits |= (meta >> 5) << (meta & 31u);
-                    }
-                }
-            }
-            cur = make_uint2(q1.x, (ihits << 24) | imask);
-            // descend into the nearest child first (closest hit: best_t
-            // shrinks sooner; any hit: near occluders are found sooner); the
-            // other hit children wait on the stack as one group
-            if (ihits & (ihits - 1u)) {
-                stk.store(sp, make_uint2(q1.x, ((ihits & ~(1u << near_k)) << 24) | imask));
-                sp++;
-                cur.y = (1u << (24 + near_k)) | imask;
-            }
-            tri = make_uint2(q1.y, thits);
-        } else {
+        if (!(cur.y & 0xff000000u)) {
             if (sp == 0) return true;
             sp--;
-            cur = stk.load(sp);
-            return false;
+            cur = stk.load(sp);  // pushed groups always hold a hit child
         }
+        const uint32_t hits = cur.y >> 24;
+        const int k = __builtin_ctz(hits);
+        cur.y &= ~(1u << (24 + k));
+        const int slot = k ^ oct;
+        const uint32_t node_idx = cur.x + __builtin_popcount(cur.y & 0xffu & ((1u << slot) - 1u));
+        if (cur.y & 0xff000000u) {
+            stk.store(sp, cur);
+            sp++;
+        }
+        const uint4* np = reinterpret_cast<const uint4*>(nodes + node_idx);
+        const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
+        if (STATS) n_nodes++;
+        const float px = __uint_as_float(q0.x), py = __uint_as_float(q0.y), pz = __uint_as_float(q0.z);
+        const float sx = exp_scale((uint8_t)(q0.w & 0xffu));
+        const float sy = exp_scale((uint8_t)((q0.w >> 8) & 0xffu));
+        const float sz = exp_scale((uint8_t)((q0.w >> 16) & 0xffu));
+        const uint32_t imask = q0.w >> 24;
+        // near and far planes per axis for the lane's direction: the slab
+        // test's min/max pairs, chosen once per node (t is monotonic in the
+        // plane, so the values are those of min/max)
+        const bool fx = inv.x < 0.0f, fy = inv.y < 0.0f, fz = inv.z < 0.0f;
+        const uint32_t nx0 = fx ? q3.z : q2.x, nx1 = fx ? q3.w : q2.y, xx0 = fx ? q2.x : q3.z, xx1 = fx ? q2.y : q3.w;
+        const uint32_t ny0 = fy ? q4.x : q2.z, ny1 = fy ? q4.y : q2.w, xy0 = fy ? q2.z : q4.x, xy1 = fy ? q2.w : q4.y;
+        const uint32_t nz0 = fz ? q4.z : q3.x, nz1 = fz ? q4.w : q3.y, xz0 = fz ? q3.x : q4.z, xz1 = fz ? q3.y : q4.w;
+        uint32_t ihits = 0, thits = 0;
+        float near_t = kInf;
+        int near_k = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const uint32_t meta = byte_of(q1.z, q1.w, c);
+            if (meta == 0) continue;
+            const float tnx = (decode_q(px, (uint8_t)byte_of(nx0, nx1, c), sx) - o.x) * inv.x;
+            const float tfx = (decode_q(px, (uint8_t)byte_of(xx0, xx1, c), sx) - o.x) * inv.x;
+            const float tny = (decode_q(py, (uint8_t)byte_of(ny0, ny1, c), sy) - o.y) * inv.y;
+            const float tfy = (decode_q(py, (uint8_t)byte_of(xy0, xy1, c), sy) - o.y) * inv.y;
+            const float tnz = (decode_q(pz, (uint8_t)byte_of(nz0, nz1, c), sz) - o.z) * inv.z;
+            const float tfz = (decode_q(pz, (uint8_t)byte_of(xz0, xz1, c), sz) - o.z) * inv.z;
+            const float tn = max_(max_(tnx, tny), max_(tnz, tmin));
+            const float tf = min_(min_(tfx, tfy), min_(tfz, best_t)) * kBoxSlack;
+            if (tn <= tf) {
+                if ((imask >> c) & 1u) {
+                    ihits |= 1u << (c ^ oct);
+                    if (tn < near_t) {
+                        near_t = tn;
+                        near_k = c ^ oct;
+                    }
+                } else {
+                    thits |= (meta >> 5) << (meta & 31u);
+                }
+            }
+        }
+        cur = make_uint2(q1.x, (ihits << 24) | imask);
+        // descend into the nearest child first (closest hit: best_t
+        // shrinks sooner; any hit: near occluders are found sooner); the
+        // other hit children wait on the stack as one group
+        if (ihits & (ihits - 1u)) {
+            stk.store(sp, make_uint2(q1.x, ((ihits & ~(1u << near_k)) << 24) | imask));
+            sp++;
+            cur.y = (1u << (24 + near_k)) | imask;
+        }
+        uint2 tri = make_uint2(q1.y, thits);
         while (tri.y) {
             const int b = __builtin_ctz(tri.y);
             tri.y &= tri.y - 1u;
