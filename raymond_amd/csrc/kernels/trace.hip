@@ -56,6 +56,7 @@ struct RayState {
     float bb0, bb1;
     bool found;
     uint2 cur;  // node group: child base, (remaining child hits << 24) | internal mask
+    uint2 tg;   // postponed leaf triangles: base, mask
     int sp;
     uint32_t n_nodes, n_tris;
 
@@ -75,16 +76,30 @@ struct RayState {
         bb0 = 0.0f;
         bb1 = 0.0f;
         found = false;
+        tg = make_uint2(0u, 0u);
         n_nodes = 0;
         n_tris = 0;
     }
 
-    // One node and the triangles of its leaf children (after a stack pop
-    // when the current group is used up, so that no step is a bare pop).
-    // Returns true when the traversal has ended (ANY: also at the first hit
-    // in (tmin, tmax)).
+    // One node (after a stack pop when the current group is used up, so
+    // that no step is a bare pop), or the postponed triangles of the leaves
+    // hit so far.  Returns true when the traversal has ended (ANY: also at
+    // the first hit in (tmin, tmax)).
     template <bool ANY, bool STATS>
     __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, const StackRef& stk) {
+        // Leaf triangles wait until at least a quarter of the wave's active
+        // lanes have some (or no lane can take a node), then those lanes test
+        // them together instead of each node step running a loop most lanes
+        // sit out.  (A quarter measured best against a half, an eighth and a
+        // sixteenth on the 5.2M-triangle sphere field.)
+        const bool has_tris = tg.y != 0u;
+        const bool node_work = (cur.y & 0xff000000u) || sp > 0;
+        const unsigned long long want = __ballot(has_tris);
+        if (__popcll(want) * 4 >= __popcll(__ballot(true)) || __ballot(node_work && !has_tris) == 0ull) {
+            if (!has_tris) return !node_work;
+            return test_tris<ANY, STATS>(tris) || (!node_work && tg.y == 0u);
+        }
+        if (has_tris) return false;
         if (!(cur.y & 0xff000000u)) {
             if (sp == 0) return true;
             sp--;
@@ -150,11 +165,17 @@ struct RayState {
             sp++;
             cur.y = (1u << (24 + near_k)) | imask;
         }
-        uint2 tri = make_uint2(q1.y, thits);
-        while (tri.y) {
-            const int b = __builtin_ctz(tri.y);
-            tri.y &= tri.y - 1u;
-            const uint4* tp = reinterpret_cast<const uint4*>(tris + tri.x + b);
+        tg = make_uint2(q1.y, thits);
+        return false;
+    }
+
+    // Tests the postponed triangles; true when an any-hit query found one.
+    template <bool ANY, bool STATS>
+    __device__ bool test_tris(const BvhTri* __restrict__ tris) {
+        while (tg.y) {
+            const int b = __builtin_ctz(tg.y);
+            tg.y &= tg.y - 1u;
+            const uint4* tp = reinterpret_cast<const uint4*>(tris + tg.x + b);
             const uint4 a = tp[0], bq = tp[1], cq = tp[2];
             if (STATS) n_tris++;
             const vec3 v0(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
