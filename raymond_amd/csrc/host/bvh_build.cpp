@@ -1,6 +1,7 @@
-// Host BVH8 builder: binned-SAH binary BVH, greedy collapse to 8 children by
-// largest surface area, octant slot assignment and conservative 8-bit
-// quantisation (layout in device/bvh.h).
+// Host BVH8 builder: binned-SAH binary BVH, SAH-optimal collapse to 8-wide
+// nodes by dynamic programming over the number of child slots a subtree may
+// take (the collapse of Ylitie, Karras and Laine, HPG 2017), octant slot
+// assignment and conservative 8-bit quantisation (layout in device/bvh.h).
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -197,6 +198,88 @@ bool quantise(Bvh8Node& out, const Aabb& node_box, const Aabb* child, const bool
     return true;
 }
 
+// Collapse costs: a node visit tests 8 boxes, a triangle one watertight
+// test.  0.6 measured best on the 5.2M-triangle sphere field among 0.15,
+// 0.3, 0.6 and 1.0 (within 2% of each other).
+constexpr double kCostNode = 1.0;
+constexpr double kCostTri = 0.6;
+
+// cost[8 n + i - 1]: least SAH cost of BVH2 subtree n spread over at most i
+// child slots of its BVH8 parent.  how[8 n + i - 1]: for i = 1, 0 = one leaf
+// slot, 1 = one internal child; for i > 1, k in 1..i-1 = k slots to the left
+// child and i - k to the right, or 0 = the best of i - 1 slots.
+struct Collapse {
+    std::vector<float> cost;
+    std::vector<uint8_t> how;
+    std::vector<uint32_t> first, count;  // primitive range of every BVH2 node
+
+    void run(const std::vector<Node2>& N2, int n2) {
+        cost.assign((size_t)n2 * 8, 0.0f);
+        how.assign((size_t)n2 * 8, 0);
+        first.assign(n2, 0);
+        count.assign(n2, 0);
+        // children have larger indices than their parents
+        for (int n = n2 - 1; n >= 0; n--) {
+            const Node2& nd = N2[n];
+            const double area = nd.box.area();
+            float* c = &cost[(size_t)n * 8];
+            uint8_t* h = &how[(size_t)n * 8];
+            if (nd.left < 0) {
+                first[n] = nd.first;
+                count[n] = nd.count;
+                for (int i = 0; i < 8; i++) {
+                    c[i] = (float)(area * kCostTri * nd.count);
+                    h[i] = 0;
+                }
+                continue;
+            }
+            first[n] = first[nd.left];
+            count[n] = count[nd.left] + count[nd.right];
+            const float* cl = &cost[(size_t)nd.left * 8];
+            const float* cr = &cost[(size_t)nd.right * 8];
+            // split of i slots between the children, i = 2..8
+            double dist[9];
+            uint8_t dk[9];
+            for (int i = 2; i <= 8; i++) {
+                dist[i] = INFINITY;
+                dk[i] = 1;
+                for (int k = 1; k < i; k++) {
+                    const double v = (double)cl[k - 1] + (double)cr[i - k - 1];
+                    if (v < dist[i]) {
+                        dist[i] = v;
+                        dk[i] = (uint8_t)k;
+                    }
+                }
+            }
+            const double internal = area * kCostNode + dist[8];
+            const double leaf = count[n] <= (uint32_t)kMaxLeaf ? area * kCostTri * count[n] : INFINITY;
+            c[0] = (float)std::min(internal, leaf);
+            h[0] = leaf <= internal ? 0 : 1;
+            for (int i = 2; i <= 8; i++) {
+                if (dist[i] < (double)c[i - 2]) {
+                    c[i - 1] = (float)dist[i];
+                    h[i - 1] = dk[i];
+                } else {
+                    c[i - 1] = c[i - 2];
+                    h[i - 1] = 0;
+                }
+            }
+        }
+    }
+
+    // The child slots of BVH2 node n given i slots: (node, is_leaf).
+    void unpack(const std::vector<Node2>& N2, int n, int i, std::vector<std::pair<int, bool>>& out) const {
+        while (i > 1 && how[(size_t)n * 8 + i - 1] == 0) i--;
+        if (i == 1) {
+            out.push_back({n, N2[n].left < 0 || how[(size_t)n * 8] == 0});
+            return;
+        }
+        const int k = how[(size_t)n * 8 + i - 1];
+        unpack(N2, N2[n].left, k, out);
+        unpack(N2, N2[n].right, i - k, out);
+    }
+};
+
 }  // namespace
 
 Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error) {
@@ -237,7 +320,10 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
     res.bvh2_nodes = b2.next.load();
     const std::vector<Node2>& N2 = b2.nodes;
 
-    // Collapse: each BVH8 node lists up to 8 BVH2 nodes as children.
+    // Collapse: each BVH8 node lists up to 8 BVH2 subtrees as children, as
+    // the cost table distributes them.
+    Collapse col;
+    col.run(N2, res.bvh2_nodes);
     struct Pending {
         int n2;      // BVH2 node this BVH8 node expands
         int index;   // BVH8 node index
@@ -250,33 +336,36 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
     queue.push_back({0, 0, 1});
     size_t head = 0;
     int max_depth = 1;
+    std::vector<std::pair<int, bool>> slots;
     while (head < queue.size()) {
         const Pending pd = queue[head++];
         max_depth = std::max(max_depth, pd.depth);
         int ch[8];
+        bool leaf[8];
         int nc = 0;
-        const Node2& root2 = N2[pd.n2];
-        if (root2.left < 0) {
-            ch[nc++] = pd.n2;
+        slots.clear();
+        if (N2[pd.n2].left < 0 || (pd.n2 == 0 && col.how[0] == 0)) {
+            slots.push_back({pd.n2, true});
         } else {
-            ch[nc++] = root2.left;
-            ch[nc++] = root2.right;
-            while (nc < 8) {
-                int best = -1;
-                double best_area = -1.0;
-                for (int c = 0; c < nc; c++) {
-                    if (N2[ch[c]].left < 0) continue;
-                    const double a = N2[ch[c]].box.area();
-                    if (a > best_area) {
-                        best_area = a;
-                        best = c;
-                    }
+            // the node's own 8 slots: the best split of 8 between its children
+            int best_k = 1;
+            double best = INFINITY;
+            const float* cl = &col.cost[(size_t)N2[pd.n2].left * 8];
+            const float* cr = &col.cost[(size_t)N2[pd.n2].right * 8];
+            for (int k = 1; k < 8; k++) {
+                const double v = (double)cl[k - 1] + (double)cr[8 - k - 1];
+                if (v < best) {
+                    best = v;
+                    best_k = k;
                 }
-                if (best < 0) break;
-                const Node2& e = N2[ch[best]];
-                ch[best] = e.left;
-                ch[nc++] = e.right;
             }
+            col.unpack(N2, N2[pd.n2].left, best_k, slots);
+            col.unpack(N2, N2[pd.n2].right, 8 - best_k, slots);
+        }
+        for (const auto& sl : slots) {
+            ch[nc] = sl.first;
+            leaf[nc] = sl.second;
+            nc++;
         }
         // octant slot assignment: slot k takes the child nearest corner k
         Aabb node_box;
@@ -315,7 +404,7 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
             if (slot_child[s] < 0) continue;
             used[s] = true;
             cb[s] = N2[ch[slot_child[s]]].box;
-            if (N2[ch[slot_child[s]]].left >= 0) n_internal++;
+            if (!leaf[slot_child[s]]) n_internal++;
         }
         if (!quantise(out, node_box, cb, used)) {
             if (error) *error = "BVH8 quantisation failed";
@@ -327,17 +416,17 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
         int rank = 0;
         for (int s = 0; s < 8; s++) {
             if (slot_child[s] < 0) continue;
-            const Node2& c2 = N2[ch[slot_child[s]]];
-            if (c2.left >= 0) {
+            const int c2 = ch[slot_child[s]];
+            if (!leaf[slot_child[s]]) {
                 out.imask |= (uint8_t)(1u << s);
                 out.meta[s] = (uint8_t)((1u << 5) | (24u + s));
-                queue.push_back({ch[slot_child[s]], (int)out.child_base + rank, pd.depth + 1});
+                queue.push_back({c2, (int)out.child_base + rank, pd.depth + 1});
                 rank++;
             } else {
-                const uint32_t cnt = c2.count;
+                const uint32_t cnt = col.count[c2];
                 out.meta[s] = (uint8_t)((((1u << cnt) - 1u) << 5) | tri_off);
                 for (uint32_t t = 0; t < cnt; t++) {
-                    const BuildTri& bt = tris[idx[c2.first + t]];
+                    const BuildTri& bt = tris[idx[col.first[c2] + t]];
                     BvhTri r;
                     r.v0x = bt.v0.x; r.v0y = bt.v0.y; r.v0z = bt.v0.z; r.inst = bt.inst;
                     r.v1x = bt.v1.x; r.v1y = bt.v1.y; r.v1z = bt.v1.z; r.prim = bt.prim;
