@@ -221,7 +221,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--spp-per-launch", type=int, default=1, help="path-queue capacity in frames")
+    ap.add_argument("--spp-per-launch", type=int, default=0,
+                    help="path-queue capacity in frames (0: about 2M paths in flight, at least one frame)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
@@ -290,7 +291,9 @@ def main():
           (setup_s, sst.triangles, sst.bvh_nodes, sst.bvh_depth, sst.bvh_build_seconds, sst.jit_seconds,
            sst.envmap_seconds), file=sys.stderr, flush=True)
 
-    spl = args.spp_per_launch
+    # small images keep several samples per pixel in flight so that each
+    # bounce-loop iteration fills the GPU
+    spl = args.spp_per_launch if args.spp_per_launch > 0 else max(1, (1 << 21) // (W * H))
     r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
     r.set_uniforms(timing=0 if args.graph else 1, rr_depth=rr_depth, material_sort=sort, use_graph=args.graph)
     if args.warmup:

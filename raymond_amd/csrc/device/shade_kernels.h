@@ -387,12 +387,20 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
     }
 }
 
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_kernel(ShadeArgs a) { shade_queue<false, false>(a); }
+// At least 4 waves per SIMD: large inlined graph sets would otherwise take
+// ~250 VGPRs (one or two waves); the spills this forces cost less than the
+// latency it hides (32 random 20-60 node graphs at 1080p: 1450 -> 1583
+// Mrays/s; 3 waves: 1559).
+constexpr int kShadeMinWaves = 4;
+
+extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_kernel(ShadeArgs a) {
+    shade_queue<false, false>(a);
+}
 
 #if defined(RM_CACHE_KERNEL)
 // Renderers with a radiance cache: lookups, training paths and their
 // radiance sums.  Compiled only into the modules of scenes a cache renders.
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_cache_kernel(ShadeArgs a) {
+extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_cache_kernel(ShadeArgs a) {
     shade_queue<false, true>(a);
 }
 #endif
@@ -400,7 +408,7 @@ extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_cache_kernel(
 #if defined(RM_BATCH_KERNEL)
 // Batched jobs (a.batch.jobs non-null): per-job lights, depth and path counts.
 // Compiled only into the modules of batches.
-extern "C" __global__ __launch_bounds__(kShadeBlock) void rm_shade_batch_kernel(ShadeArgs a) {
+extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_batch_kernel(ShadeArgs a) {
     shade_queue<true, false>(a);
 }
 #endif
