@@ -109,6 +109,13 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def queue_frames(args, W, H):
+    """Samples per pixel in flight: about 16M paths, so that every
+    bounce-loop iteration fills the GPU and the drain of the last paths is
+    spread over many samples (1080p: 8 frames; 512x512: 64)."""
+    return args.spp_per_launch if args.spp_per_launch > 0 else max(1, (1 << 24) // (W * H))
+
+
 def bench_cache(args):
     """Configuration 5: the Cornell box at full HD, depth 12, roulette from
     bounce 3.  A converged FP64 render of the plain tracer is the reference;
@@ -136,8 +143,10 @@ def bench_cache(args):
     print("reference: %d spp in %.1fs (%.1f Mrays/s)" % (args.ref_spp, ref_s, st.total_rays / ref_s / 1e6),
           file=sys.stderr, flush=True)
 
+    spl = queue_frames(args, W, H)
+
     def run(cached):
-        r = api.Renderer(scene, W, H, max_depth=D)
+        r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
         r.set_uniforms(rr_depth=3)
         cache = None
         def new_cache():
@@ -222,7 +231,7 @@ def main():
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--spp-per-launch", type=int, default=0,
-                    help="path-queue capacity in frames (0: about 2M paths in flight, at least one frame)")
+                    help="path-queue capacity in frames (0: about 16M paths in flight, at least one frame)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
@@ -291,9 +300,7 @@ def main():
           (setup_s, sst.triangles, sst.bvh_nodes, sst.bvh_depth, sst.bvh_build_seconds, sst.jit_seconds,
            sst.envmap_seconds), file=sys.stderr, flush=True)
 
-    # small images keep several samples per pixel in flight so that each
-    # bounce-loop iteration fills the GPU
-    spl = args.spp_per_launch if args.spp_per_launch > 0 else max(1, (1 << 21) // (W * H))
+    spl = queue_frames(args, W, H)
     r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
     r.set_uniforms(timing=0 if args.graph else 1, rr_depth=rr_depth, material_sort=sort, use_graph=args.graph)
     if args.warmup:

@@ -343,13 +343,13 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
     for (const JobInfo& ji : b->jobs) pixels += (uint64_t)ji.width * ji.height;
     size_t prefix = 0;
     uint64_t prefix_paths = 0;
-    const uint64_t want = std::max<uint64_t>(pixels * 2, 1ull << 23);
+    const uint64_t want = std::max<uint64_t>(pixels * 2, 1ull << 25);
     while (prefix < b->slices.size() && prefix_paths < want) {
         const JobInfo& ji = b->jobs[b->slices[prefix].job];
         prefix_paths += (uint64_t)ji.width * ji.height;
         prefix++;
     }
-    const uint32_t queues[4] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+    const uint32_t queues[4] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     uint32_t used = 0;
     double best_rate = -1.0;
     rm_batch_config base = *best;
