@@ -233,7 +233,10 @@ typedef struct rm_render_stats {
 } rm_render_stats;
 
 /* Replaces Renderer.init + setSize: samples_per_launch paths per pixel are
- * traced together in one wavefront. */
+ * traced together in one wavefront.  About 2^24 paths in flight
+ * (samples_per_launch = 2^24 / (width * height), at least 1) keeps every
+ * bounce-loop iteration large enough to fill an MI355X; the queues then take
+ * about 200 bytes per path. */
 int rm_renderer_create(rm_scene* scene, uint32_t width, uint32_t height, uint32_t max_depth,
                        uint32_t samples_per_launch, rm_renderer** out);
 void rm_renderer_destroy(rm_renderer* r);
