@@ -154,6 +154,13 @@ void bvh_box(const or_scene& s, uint32_t b, uint32_t e, BNode& n) {
                 n.hi[a] = std::max(n.hi[a], rm::comp(v, a));
             }
     }
+    // one ulp outward: a ray parallel to an axis that lies in a face of the
+    // geometry's bounds then starts strictly inside the slab, where the slab
+    // test's 0 * (1 / 1e-30) would otherwise close it at t = 0
+    for (int a = 0; a < 3; a++) {
+        n.lo[a] = std::nextafter(n.lo[a], -INFINITY);
+        n.hi[a] = std::nextafter(n.hi[a], INFINITY);
+    }
 }
 
 int bvh_build(or_scene& s, uint32_t b, uint32_t e) {

@@ -152,14 +152,18 @@ struct Builder2 {
 };
 
 // Quantisation of one node; returns false if a child box cannot be covered.
+// Decoded child boxes enclose the exact ones strictly (the grid origin sits
+// one ulp below the node box): a ray parallel to an axis that lies in a face
+// of a child's geometry bounds then starts strictly inside that slab, where
+// the slab test's 0 * (1 / 1e-30) would otherwise close it at t = 0.
 bool quantise(Bvh8Node& out, const Aabb& node_box, const Aabb* child, const bool* used) {
-    out.px = node_box.lo[0];
-    out.py = node_box.lo[1];
-    out.pz = node_box.lo[2];
+    out.px = std::nextafter(node_box.lo[0], -INFINITY);
+    out.py = std::nextafter(node_box.lo[1], -INFINITY);
+    out.pz = std::nextafter(node_box.lo[2], -INFINITY);
     const float p[3] = {out.px, out.py, out.pz};
     uint8_t e[3];
     for (int a = 0; a < 3; a++) {
-        const double ext = (double)node_box.hi[a] - (double)node_box.lo[a];
+        const double ext = (double)node_box.hi[a] - (double)p[a];
         int ee = 1;
         if (ext > 0.0) {
             ee = (int)std::ceil(std::log2(ext / 255.0)) + 127;
@@ -176,10 +180,10 @@ bool quantise(Bvh8Node& out, const Aabb& node_box, const Aabb* child, const bool
                 int qlo = (int)std::floor(rlo), qhi = (int)std::ceil(rhi);
                 qlo = std::max(0, std::min(255, qlo));
                 qhi = std::max(0, std::min(255, qhi));
-                while (qlo > 0 && decode_q(p[a], (uint8_t)qlo, scale) > child[c].lo[a]) qlo--;
-                while (qhi < 255 && decode_q(p[a], (uint8_t)qhi, scale) < child[c].hi[a]) qhi++;
-                if (decode_q(p[a], (uint8_t)qlo, scale) > child[c].lo[a] ||
-                    decode_q(p[a], (uint8_t)qhi, scale) < child[c].hi[a]) {
+                while (qlo > 0 && decode_q(p[a], (uint8_t)qlo, scale) >= child[c].lo[a]) qlo--;
+                while (qhi < 255 && decode_q(p[a], (uint8_t)qhi, scale) <= child[c].hi[a]) qhi++;
+                if (decode_q(p[a], (uint8_t)qlo, scale) >= child[c].lo[a] ||
+                    decode_q(p[a], (uint8_t)qhi, scale) <= child[c].hi[a]) {
                     ok = false;
                     break;
                 }

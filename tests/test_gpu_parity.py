@@ -88,6 +88,48 @@ def test_closest_cornell_matches_brute_force(cornell):
     assert np.mean(gpu[0] >= 0) > 0.5
 
 
+def axis_rays(n, seed):
+    """Rays with zero direction components: the six axis directions and
+    directions in the coordinate planes, from inside the box and from points
+    exactly on its wall planes (rays that lie in a wall or graze an edge)."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.02, 0.98, (n, 3))
+    on_wall = rng.random(n) < 0.5
+    axis = rng.integers(0, 3, n)
+    side = rng.integers(0, 2, n).astype(np.float64)
+    o[on_wall, axis[on_wall]] = side[on_wall]
+    d = np.zeros((n, 3))
+    kind = rng.integers(0, 2, n)
+    a1 = rng.integers(0, 3, n)
+    sign = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    d[np.arange(n), a1] = sign                              # kind 0: an axis
+    a2 = (a1 + 1 + rng.integers(0, 2, n)) % 3              # kind 1: a coordinate plane
+    ang = rng.uniform(0, 2 * np.pi, n)
+    pl = kind == 1
+    d[pl] = 0.0
+    d[pl, a1[pl]] = np.cos(ang[pl])
+    d[pl, a2[pl]] = np.sin(ang[pl])
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = 0.0
+    r[:, 4:7] = d
+    r[:, 7] = np.inf
+    return r
+
+
+def test_axis_aligned_rays_match_brute_force(cornell):
+    desc, scene, orc = cornell
+    rays = axis_rays(20000, 7)
+    assert (rays[:, 4:7] == 0.0).any(axis=1).all()
+    gpu = scene.trace_closest(rays)
+    ref = orc.trace_closest(rays, brute_force=True)
+    assert_hits_equal(gpu, ref)
+    assert np.mean(gpu[0] >= 0) > 0.5
+    short = rays.copy()
+    short[:, 7] = 0.3
+    assert np.array_equal(scene.trace_any(short), orc.trace_any(short, brute_force=True))
+
+
 def test_any_cornell_matches_brute_force(cornell):
     desc, scene, orc = cornell
     rays = random_rays(20000, 0.02, 0.98, 3, tmin=1e-3, tmax=0.5)
