@@ -74,7 +74,10 @@ def bench_batch(args):
     print("autotune %.1fs (cache %s): queue %d, sort %d; tried %s" % (time.time() - t1, hit, cfg.queue_paths,
                                                                       cfg.material_sort, tried),
           file=sys.stderr, flush=True)
-    st = batch.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort, rr_depth=3, timing=0)
+    # the bounce loop replays captured hipGraph chunks (same speed as direct
+    # launches at these queue depths: 2999 against 2998 Mrays/s)
+    st = batch.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort, rr_depth=3, timing=0,
+                      use_graph=1)
     secs = st.ms_total / 1e3
     mrays = st.total_rays / secs / 1e6
     times = batch.job_times()
@@ -91,9 +94,10 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {"workload": "batched uneven jobs (config 4): %d Cornell-box jobs (seed 4), %d triangles in "
                                "total (1k-2M each), 256^2-1024^2 pixels, 16-256 spp, depth 6-12, roulette from "
-                               "bounce 3, one regenerating bounce loop" % (len(specs), tris),
+                               "bounce 3, one regenerating bounce loop replayed from captured hipGraphs" %
+                               (len(specs), tris),
                    "jobs": len(specs), "triangles": tris, "samples": paths,
-                   "queue_paths": cfg.queue_paths, "material_sort": bool(cfg.material_sort),
+                   "queue_paths": cfg.queue_paths, "material_sort": bool(cfg.material_sort), "hipgraph": True,
                    "autotune": {"cache_hit": hit, "candidates": [
                        {"queue_paths": q, "material_sort": m, "mrays_per_s": round(r, 1)} for q, m, r in tried]}},
         "seconds": round(secs, 3),
