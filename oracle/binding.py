@@ -174,6 +174,10 @@ def _rc_lib():
     if not getattr(L, "_rc_ready", False):
         L.or_rc_round_half.restype = C.c_float
         L.or_rc_round_half.argtypes = [C.c_float]
+        L.or_rc_atan2.restype = C.c_float
+        L.or_rc_atan2.argtypes = [C.c_float, C.c_float]
+        L.or_rc_acos.restype = C.c_float
+        L.or_rc_acos.argtypes = [C.c_float]
         L.or_rc_grid.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
         L.or_rc_index.restype = C.c_uint32
         L.or_rc_index.argtypes = [C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
@@ -186,6 +190,14 @@ def _rc_lib():
 
 def rc_round_half(x):
     return np.array([_rc_lib().or_rc_round_half(float(v)) for v in np.ravel(x)], np.float32)
+
+
+def rc_atan2(y, x):
+    return _rc_lib().or_rc_atan2(y, x)
+
+
+def rc_acos(z):
+    return _rc_lib().or_rc_acos(z)
 
 
 def rc_grid(log2_table):

@@ -161,6 +161,10 @@ extern "C" {
 
 float or_rc_round_half(float f) { return round_half(f); }
 
+// The float atan2 / acos of the vertex features (rcache.h), for the tests.
+float or_rc_atan2(float y, float x) { return rm::rc_atan2(y, x); }
+float or_rc_acos(float z) { return rm::rc_acos(z); }
+
 void or_rc_grid(uint32_t log2_table, uint32_t* res, uint32_t* dense) {
     const Grid G = make_grid(log2_table);
     for (int l = 0; l < kRcLevels; l++) {

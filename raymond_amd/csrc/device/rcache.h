@@ -62,14 +62,43 @@ RM_HD void rc_cell(float p, uint32_t res, uint32_t& cell, float& frac) {
     frac = x - c;
 }
 
+// atan2 for network inputs, in float: octant reduction to [0, 1] and the
+// odd polynomial of Abramowitz and Stegun 4.4.49 (error below 2e-8 rad in
+// exact arithmetic).  The features only feed the network, so they need none
+// of the correctly rounded elementary functions the shading code uses (those
+// run in double precision and cost about ten times as much).
+RM_HD float rc_atan2(float y, float x) {
+    const float ax = x < 0.0f ? -x : x, ay = y < 0.0f ? -y : y;
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    const float t = mx > 0.0f ? mn / mx : 0.0f;
+    const float t2 = t * t;
+    float p = 0.0028662257f;
+    p = p * t2 - 0.0161657367f;
+    p = p * t2 + 0.0429096138f;
+    p = p * t2 - 0.0752896400f;
+    p = p * t2 + 0.1065626393f;
+    p = p * t2 - 0.1420889944f;
+    p = p * t2 + 0.1999355085f;
+    p = p * t2 - 0.3333314528f;
+    p = p * t2 + 1.0f;
+    float a = p * t;
+    a = ay > ax ? 1.5707963268f - a : a;
+    a = x < 0.0f ? 3.1415926536f - a : a;
+    return y < 0.0f ? -a : a;
+}
+RM_HD float rc_acos(float z) {
+    z = clamp_(z, -1.0f, 1.0f);
+    return rc_atan2(sqrt_((1.0f - z) * (1.0f + z)), z);
+}
+
 // The 9 vertex features: outgoing direction and normal in spherical
 // coordinates (scaled to [0,1]), roughness, albedo and the specular-lobe
 // probability.
 RM_HD void rc_vertex_features(vec3 wo, vec3 n, float roughness, vec3 albedo, float spec, float* f) {
-    f[0] = acos_(clamp_(wo.z, -1.0f, 1.0f)) * kInvPi;
-    f[1] = atan2_(wo.y, wo.x) * (0.5f * kInvPi) + 0.5f;
-    f[2] = acos_(clamp_(n.z, -1.0f, 1.0f)) * kInvPi;
-    f[3] = atan2_(n.y, n.x) * (0.5f * kInvPi) + 0.5f;
+    f[0] = rc_acos(wo.z) * kInvPi;
+    f[1] = rc_atan2(wo.y, wo.x) * (0.5f * kInvPi) + 0.5f;
+    f[2] = rc_acos(n.z) * kInvPi;
+    f[3] = rc_atan2(n.y, n.x) * (0.5f * kInvPi) + 0.5f;
     f[4] = roughness;
     f[5] = saturate(albedo.x);
     f[6] = saturate(albedo.y);

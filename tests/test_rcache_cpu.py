@@ -189,3 +189,17 @@ def test_gradients_match_central_differences():
         tb = table.copy()
         tb[k] = 0.375
         assert check(mlp, tb, "table", k) != 0.0
+
+
+def test_feature_atan2_and_acos_against_numpy():
+    """The float atan2/acos of the vertex features (Abramowitz and Stegun
+    4.4.49 after octant reduction) against numpy over all quadrants, axes and
+    the clamp of acos."""
+    rng = np.random.default_rng(5)
+    y = np.concatenate([rng.normal(size=4000), [0.0, 0.0, 1.0, -1.0, 0.0, 1e-30]]).astype(np.float32)
+    x = np.concatenate([rng.normal(size=4000), [1.0, -1.0, 0.0, 0.0, 0.0, -1.0]]).astype(np.float32)
+    got = np.array([binding.rc_atan2(float(a), float(b)) for a, b in zip(y, x)])
+    assert np.abs(got - np.arctan2(y.astype(np.float64), x.astype(np.float64))).max() < 2e-6
+    z = np.concatenate([np.linspace(-1, 1, 2001), [1.5, -1.5]]).astype(np.float32)
+    got = np.array([binding.rc_acos(float(v)) for v in z])
+    assert np.abs(got - np.arccos(np.clip(z.astype(np.float64), -1, 1))).max() < 2e-3
