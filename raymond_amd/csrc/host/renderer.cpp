@@ -206,7 +206,7 @@ static void prepare_cache(rm_renderer* r, uint32_t samples, uint32_t par) {
     }
 }
 
-static void CacheOverlapInit(rm_renderer* r) {
+static void init_cache_overlap(rm_renderer* r) {
     auto& o = r->overlap;
     if (o.stream) return;
     RM_HIP(hipStreamCreateWithFlags(&o.stream, hipStreamNonBlocking));
@@ -443,7 +443,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         c->queries += cache_ctr[0];
         hipStream_t cs = st;
         if (overlap) {
-            CacheOverlapInit(r);
+            init_cache_overlap(r);
             cs = r->overlap.stream;
             RM_HIP(hipEventRecord(r->overlap.loop_end, st));
             RM_HIP(hipStreamWaitEvent(cs, r->overlap.loop_end, 0));
