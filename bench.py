@@ -1,16 +1,16 @@
 """Benchmark: Mrays/s of the wavefront path tracer on one MI355X.
 
-Default workload (configuration 2): 64 level-6 icospheres displaced by 3-octave value
+Default workload, the sphere field: 64 level-6 icospheres displaced by 3-octave value
 noise plus a ground plane (5,242,882 triangles), Principled materials, an
 analytic sky importance-sampled through a 2048^2 envmap, 1920x1080, path
 depth 12.  One step = one sample per pixel (2,073,600 paths) traced through
 all bounces.  Mrays/s counts closest-hit plus shadow rays traced, the
 reference's own accounting (host/Renderer.swift:190-196,249-253).
 
-`--workload graphs` runs configuration 3 instead: the Cornell-box geometry
+`--workload graphs` runs the shader-graph workload instead: the Cornell-box geometry
 with 32 random node-graph materials, depth 8, shading sorted by material.
 
-Prints one JSON line.  `--gpus` above 1 is rejected: this build runs on one GPU.
+Prints one JSON line.  `--gpus` above 1 is rejected: the tracer runs on one GPU.
 """
 import argparse
 import json
@@ -39,18 +39,18 @@ def copy_probe(mib):
 
 def workload_name(graphs, W, H, D, sort, cornell=False):
     if cornell:
-        return ("Cornell box (config 1): 36 triangles, emitter 15, %dx%d, depth %d, roulette from bounce 3, "
+        return ("Cornell box: 36 triangles, emitter 15, %dx%d, depth %d, roulette from bounce 3, "
                 "plain tracer, 1 spp per step" % (W, H, D))
     if graphs:
-        return ("shader graphs (config 3): Cornell-box geometry, 32 random 20-60 node Cycles graphs (seed 3) "
+        return ("shader graphs: Cornell-box geometry, 32 random 20-60 node Cycles graphs (seed 3) "
                 "assigned per triangle, %dx%d, depth %d, plain tracer, 1 spp per step%s" %
                 (W, H, D, ", shading sorted by material" if sort else ""))
-    return ("sphere field (config 2): 64 noise-displaced level-6 icospheres + ground, 5,242,882 "
+    return ("sphere field: 64 noise-displaced level-6 icospheres + ground, 5,242,882 "
             "triangles, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step" % (W, H, D))
 
 
 def bench_batch(args):
-    """Configuration 4: 64 Cornell-box jobs (seed 4) with 1k-2M triangles,
+    """Batched uneven jobs: 64 Cornell-box jobs (seed 4) with 1k-2M triangles,
     256^2-1024^2 pixels, 16-256 spp and depth 6-12, rendered as one batch: one
     regenerating bounce loop over all jobs' samples, queue depth and material
     sort picked by the autotuner.  Reports aggregate Mrays/s and each job's
@@ -92,7 +92,7 @@ def bench_batch(args):
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched uneven jobs (config 4): %d Cornell-box jobs (seed 4), %d triangles in "
+        "config": {"workload": "batched uneven jobs: %d Cornell-box jobs (seed 4), %d triangles in "
                                "total (1k-2M each), 256^2-1024^2 pixels, 16-256 spp, depth 6-12, roulette from "
                                "bounce 3, one regenerating bounce loop replayed from captured hipGraphs" %
                                (len(specs), tris),
@@ -121,7 +121,7 @@ def queue_frames(args, W, H):
 
 
 def bench_cache(args):
-    """Configuration 5: the Cornell box at full HD, depth 12, roulette from
+    """Radiance cache against the plain tracer: the Cornell box at full HD, depth 12, roulette from
     bounce 3.  A converged FP64 render of the plain tracer is the reference;
     the plain tracer and the cached tracer (online training on 1 path in 16,
     cold start) each render `steps` spp, and the RMSE against the reference is
@@ -216,7 +216,7 @@ def bench_cache(args):
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "radiance cache (config 5): Cornell box %dx%d, depth %d, roulette from bounce 3, "
+        "config": {"workload": "radiance cache: Cornell box %dx%d, depth %d, roulette from bounce 3, "
                                "hash grid 16x2 FP16 x 2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 "
                                "trains, %d training steps of %d samples per spp; reference: plain tracer, %d spp, "
                                "FP64 accumulation" % (W, H, D, args.cache_steps, args.cache_batch, args.ref_spp)},
@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=16, help="oracle sample: frames (spp) of the full image")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU (the path tracer is single-GPU in this build); "
+        print("bench.py runs on exactly one GPU (the path tracer is single-GPU); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -282,7 +282,7 @@ def main():
         W, H = 512, 512
     D = args.depth or (8 if graphs or cornell else 12)
     sort = args.material_sort if args.material_sort >= 0 else int(graphs)
-    # configuration 1 runs roulette from bounce 3; the sphere field and graph
+    # the Cornell-box workload runs roulette from bounce 3; the sphere field and graph
     # scenes the reference's way, from the first bounce
     rr_depth = 3 if cornell else 0
     t0 = time.time()

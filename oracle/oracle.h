@@ -21,7 +21,7 @@
  * tests/): the device function library under raymond_amd/csrc/device
  * (RNG, warps, BSDF lobes, lights, node library, elementary functions) and
  * the watertight triangle / conservative box tests of isect.h, which define
- * this build's hit semantics (the reference's are Metal's and unpublished).
+ * the library's hit semantics (the reference's are Metal's and unpublished).
  */
 #ifndef RAYMOND_AMD_ORACLE_H
 #define RAYMOND_AMD_ORACLE_H

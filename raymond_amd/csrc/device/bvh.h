@@ -1,4 +1,4 @@
-// Compressed 8-wide BVH node and leaf-triangle records (layout of this build).
+// Compressed 8-wide BVH node and leaf-triangle records (this library's layout).
 //
 // Node: 80 bytes (five 16-byte loads).  Child boxes are quantised to 8 bits
 // per axis on a per-node grid: decoded = p + q * 2^(e-127), with q*2^(e-127)

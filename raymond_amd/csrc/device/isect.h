@@ -3,7 +3,7 @@
 //
 // The reference delegates intersection to Metal's closed intersector
 // (device/kernels/shading/raytrace.metal:24-42), whose arithmetic is not
-// published.  This build defines its own:
+// published.  This library defines its own:
 //   * triangles: the watertight test of Woop, Benthin and Wald (JCGT 2013):
 //     shear to the dominant axis, edge functions in float with a double
 //     re-evaluation when any is exactly zero, so no ray leaks between

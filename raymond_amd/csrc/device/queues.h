@@ -85,7 +85,7 @@ struct Counters {
 // counter holds.
 RM_HD uint32_t iteration_of(const uint32_t* iter_dev, uint32_t iter) { return iter_dev ? *iter_dev + iter : iter; }
 
-// Render-time settings (bridge/Uniforms.hpp:29-47 plus this build's knobs).
+// Render-time settings (bridge/Uniforms.hpp:29-47 plus this library's knobs).
 struct FrameUniforms {
     uint32_t width, height;
     uint32_t samples;          // samples per pixel in this stream

@@ -335,7 +335,7 @@ def unit_cube(material):
 
 
 def cornell_box(width=512, height=512):
-    """Config 1: unit Cornell box, 34 triangles + a 0.5x0.5 emitter (2 triangles), z up."""
+    """Unit Cornell box, 34 triangles + a 0.5x0.5 emitter (2 triangles), z up."""
     d = SceneDescription("cornell")
     WHITE, RED, GREEN, TALL, LIGHT = 0, 1, 2, 3, 4
     d.surface_materials = [
@@ -370,7 +370,7 @@ def cornell_box(width=512, height=512):
 
 
 def tessellated_cornell_box(width, height, triangles):
-    """The config-1 Cornell box with walls and boxes tessellated so that the
+    """The Cornell box of cornell_box() with walls and boxes tessellated so that the
     scene has about `triangles` triangles (the emitter stays one quad)."""
     d = cornell_box(width, height)
     k = max(1, int(round((triangles / 34.0) ** 0.5)))
@@ -461,7 +461,7 @@ def lamp_box(width=64, height=64):
 
 
 def batch_jobs(n=64, seed=4):
-    """Config 4: n Cornell-box jobs with triangle counts log-uniform in
+    """n Cornell-box jobs with triangle counts log-uniform in
     [1k, 2M], resolutions from {256, 512, 768, 1024}^2, 16-256 spp and depth
     6-12.  Returns [(scene description, width, height, samples, depth)]."""
     rng = np.random.default_rng(seed)
@@ -546,7 +546,7 @@ def vertex_normals(v, f):
 
 
 def sphere_field(width=1920, height=1080, grid=8, level=6, radius=0.4, amplitude=0.05):
-    """Config 2: grid x grid displaced icospheres (seed 1000+i), jittered grid (seed 7),
+    """grid x grid displaced icospheres (seed 1000+i), jittered grid (seed 7),
     Principled per instance (seed 42), ground plane, analytic sky."""
     d = SceneDescription("spheres")
     rng_mat = np.random.default_rng(42)
@@ -595,7 +595,7 @@ def build_device_scene(device, desc, threads=0, envmap=True, noinline=None):
 
 
 # ---------------------------------------------------------------------------
-# Config 3: seeded random node graphs
+# Seeded random node graphs
 
 
 def _node(t, inputs, params=None):
@@ -754,7 +754,7 @@ def _random_graph_attempt(name, rng, target):
 
 
 def shader_graph_scene(width=1920, height=1080, materials=32, seed=3):
-    """Config 3: config-1 geometry, `materials` random node graphs assigned per triangle."""
+    """The Cornell-box geometry, `materials` random node graphs assigned per triangle."""
     d = cornell_box(width, height)
     rng = np.random.default_rng(seed)
     graphs = [random_graph_material("graph %02d" % i, rng) for i in range(materials)]

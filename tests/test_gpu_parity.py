@@ -336,3 +336,30 @@ def test_batch_of_lamp_scenes_matches_solo(device, lamps):
         r.execute(spp)
         ref = r.image()
         assert np.abs(b.image(j) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-6), j
+
+
+def _hable_partial(x):
+    A, B, C, D, E, F = (np.float32(v) for v in (0.15, 0.50, 0.10, 0.20, 0.02, 0.30))
+    return (x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F) - E / F
+
+
+@pytest.mark.parametrize("tonemap", [0, 1, 2])
+def test_resolve_matches_normalize_image(cornell, tonemap):
+    """rm_resolve against a float32 restatement of normalizeImage
+    (device/kernels/utils/normalizeImage.metal:11-59): exposure, division by the
+    accumulated sample count, then linear / Hable / ACES."""
+    desc, scene, orc = cornell
+    r = api.Renderer(scene, 64, 64, max_depth=4)
+    r.set_uniforms(exposure=0.7, tonemapping=tonemap)
+    r.execute(3)
+    raw = r.image()[..., :3].astype(np.float32)
+    c = (np.float32(0.7) * raw) / np.float32(3.0)
+    if tonemap == 1:
+        c = _hable_partial(c * np.float32(2.0)) * (np.float32(1.0) / _hable_partial(np.float32(11.2)))
+    elif tonemap == 2:
+        a, b, cc, d, e = (np.float32(v) for v in (2.51, 0.03, 2.43, 0.59, 0.14))
+        c = np.clip((c * (a * c + b)) / (c * (cc * c + d) + e), 0.0, 1.0)
+    out = r.resolve()
+    assert np.all(out[..., 3] == 1.0)
+    assert np.allclose(out[..., :3], c, rtol=2e-6, atol=1e-7)
+    assert c.max() > 0.05
