@@ -363,3 +363,23 @@ def test_resolve_matches_normalize_image(cornell, tonemap):
     assert np.all(out[..., 3] == 1.0)
     assert np.allclose(out[..., :3], c, rtol=2e-6, atol=1e-7)
     assert c.max() > 0.05
+
+
+def test_stage_timers(cornell):
+    """timing=1 records per-stage GPU times (the reference's stage timers,
+    Renderer.swift) without changing what is rendered."""
+    desc, scene, orc = cornell
+    out = []
+    for timing in (0, 1):
+        r = api.Renderer(scene, 64, 64, max_depth=8, samples_per_launch=2)
+        r.set_uniforms(timing=timing)
+        st = r.execute(4)
+        out.append((st, r.image()))
+    (st0, img0), (st1, img1) = out
+    assert st0.total_rays == st1.total_rays and st0.iterations == st1.iterations
+    assert np.abs(img0 - img1).max() <= 1e-5 * max(np.abs(img0).max(), 1e-6)
+    assert st0.ms_trace_closest == 0.0 and st0.ms_shade == 0.0
+    for v in (st1.ms_trace_closest, st1.ms_shade, st1.ms_trace_any):
+        assert v > 0.0
+    assert st1.ms_trace_closest + st1.ms_shade + st1.ms_trace_any <= st1.ms_total
+    assert st1.trace_closest_launches == st1.iterations
