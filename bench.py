@@ -248,8 +248,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
     ap.add_argument("--cache-batch", type=int, default=8192, help="cache workload: samples per training step")
     ap.add_argument("--cache-steps", type=int, default=2, help="cache workload: training steps per sample")
-    ap.add_argument("--material-sort", type=int, default=-1,
-                    help="group hits by material before shading (-1 = on for graphs, off for spheres)")
+    ap.add_argument("--material-sort", type=int, default=1,
+                    help="group hits by material (misses in a bin of their own) before shading")
     ap.add_argument("--inline-materials", type=int, default=-1,
                     help="1: material graphs inlined into the shading kernel, 0: out-of-line calls, -1: by size")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -281,7 +281,10 @@ def main():
     if cornell and (W, H) == (1920, 1080):
         W, H = 512, 512
     D = args.depth or (8 if graphs or cornell else 12)
-    sort = args.material_sort if args.material_sort >= 0 else int(graphs)
+    # sorting pays for itself on every workload here: on the sphere field it
+    # splits misses (sky) from hits and metals from dielectrics (shading
+    # 0.53 -> 0.45 ms per spp for 0.05 ms of sorting)
+    sort = int(args.material_sort != 0)
     # the Cornell-box workload runs roulette from bounce 3; the sphere field and graph
     # scenes the reference's way, from the first bounce
     rr_depth = 3 if cornell else 0
