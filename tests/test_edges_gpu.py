@@ -1,0 +1,99 @@
+"""Edge cases on the GPU against the oracle: a scene without geometry (world
+only), one-pixel and odd-sized images, paths of depth 1, ray batches of
+length zero, and arguments the library must reject."""
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from oracle import binding
+
+pytestmark = pytest.mark.gpu
+
+
+def empty_scene():
+    d = scenes.SceneDescription("empty")
+    d.surface_materials = []
+    d.light_materials = [scenes.sky_world()]
+    d.world_shader = 0
+    d.camera = dict(transform=scenes.look_at((0.0, 0.0, 1.0), (0.0, 1.0, 1.05)), focal=1.5)
+    d.envmap_resolution = 64
+    return d
+
+
+@pytest.fixture(scope="module")
+def box(device):
+    desc = scenes.cornell_box(16, 16)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc)
+    orc = binding.OracleScene(desc)
+    orc.build_envmap()
+    return desc, scene, orc
+
+
+def per_sample(scene, orc, W, H, depth, frames=2, spl=1):
+    """Each sample of the GPU stream against the oracle's frame: ray counts
+    per depth equal, radiance within 1e-4 of the frame's peak."""
+    r = api.Renderer(scene, W, H, max_depth=depth, samples_per_launch=spl)
+    prev = np.zeros((H, W, 4), np.float32)
+    for frame in range(frames):
+        st = r.execute(1)
+        img = r.image()
+        sample = img - prev
+        prev = img
+        ref, res = orc.render(W, H, depth, frames=1, frame_begin=frame)
+        n = min(depth, 32)
+        assert [st.rays[d] for d in range(n)] == [res.rays[d] for d in range(n)]
+        assert [st.shadow_rays[d] for d in range(n)] == [res.shadow_rays[d] for d in range(n)]
+        assert np.abs(sample - ref).max() <= 1e-4 * max(np.abs(ref).max(), 1e-6) + 1e-6
+    return prev
+
+
+def test_scene_without_geometry_renders_the_world(device):
+    d = empty_scene()
+    scene = scenes.build_device_scene(device, d)
+    orc = binding.OracleScene(d)
+    orc.build_envmap()
+    img = per_sample(scene, orc, 7, 3, 4)
+    assert img[..., :3].min() > 0.0
+    rays = np.zeros((9, 8), np.float32)
+    rays[:, 4:7] = [0.0, 0.6, 0.8]
+    rays[:, 7] = np.inf
+    t, inst, prim, bary = scene.trace_closest(rays)
+    assert np.all(t < 0.0)
+    assert not scene.trace_any(rays).any()
+
+
+@pytest.mark.parametrize("W,H,spl", [(1, 1, 1), (7, 3, 3), (65, 9, 1)])
+def test_small_and_odd_images(box, W, H, spl):
+    """Sizes that leave partial 8x8 raygen tiles, and a queue that holds
+    several samples of a tiny image."""
+    desc, scene, orc = box
+    per_sample(scene, orc, W, H, 6, frames=3, spl=spl)
+
+
+def test_depth_one_traces_camera_rays_only(box):
+    """max_depth 1: the last depth's shadow and continuation rays are never
+    traced (Renderer.swift:623-625), so only emitters seen directly count."""
+    desc, scene, orc = box
+    img = per_sample(scene, orc, 16, 16, 1)
+    r = api.Renderer(scene, 16, 16, max_depth=1)
+    st = r.execute(1)
+    assert st.shadow_total == 0 and st.closest_total == 16 * 16
+    assert img[..., :3].max() > 0.0  # the emitter
+
+
+def test_ray_batches_of_length_zero(box):
+    desc, scene, orc = box
+    empty = np.zeros((0, 8), np.float32)
+    t, inst, prim, bary = scene.trace_closest(empty)
+    assert t.shape == (0,) and bary.shape == (0, 2)
+    assert scene.trace_any(empty).shape == (0,)
+
+
+@pytest.mark.parametrize("W,H,depth,spl", [(0, 16, 4, 1), (16, 0, 4, 1), (16, 16, 0, 1), (16, 16, 256, 1),
+                                           (16, 16, 4, 0)])
+def test_invalid_renderer_arguments_are_rejected(box, W, H, depth, spl):
+    desc, scene, orc = box
+    with pytest.raises(RuntimeError):
+        api.Renderer(scene, W, H, max_depth=depth, samples_per_launch=spl)
