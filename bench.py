@@ -77,7 +77,7 @@ def bench_batch(args):
     # the bounce loop replays captured hipGraph chunks (same speed as direct
     # launches at these queue depths: 2999 against 2998 Mrays/s)
     st = batch.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort, rr_depth=3, timing=0,
-                      use_graph=1)
+                      use_graph=1, order=api.BATCH_SHORTEST_FIRST)
     secs = st.ms_total / 1e3
     mrays = st.total_rays / secs / 1e6
     times = batch.job_times()
@@ -98,6 +98,7 @@ def bench_batch(args):
                                (len(specs), tris),
                    "jobs": len(specs), "triangles": tris, "samples": paths,
                    "queue_paths": cfg.queue_paths, "material_sort": bool(cfg.material_sort), "hipgraph": True,
+                   "order": "shortest job first",
                    "autotune": {"cache_hit": hit, "candidates": [
                        {"queue_paths": q, "material_sort": m, "mrays_per_s": round(r, 1)} for q, m, r in tried]}},
         "seconds": round(secs, 3),

@@ -341,7 +341,13 @@ typedef struct rm_batch_config {
     uint32_t rr_depth;       /* Russian roulette from this depth (0 = from the first bounce) */
     uint32_t timing;         /* per-stage GPU times */
     uint32_t use_graph;      /* replay each bounce loop from a captured hipGraph (ignored with timing) */
+    uint32_t order;          /* RM_BATCH_SHORTEST_FIRST (default): jobs in order of their path
+                              * count, so small jobs complete early and fewer jobs share the
+                              * caches at a time; RM_BATCH_SAMPLE_MAJOR: every job progresses
+                              * from the start */
 } rm_batch_config;
+
+enum { RM_BATCH_SHORTEST_FIRST = 0, RM_BATCH_SAMPLE_MAJOR = 1 };
 
 typedef struct rm_batch_stats {
     uint64_t total_rays, closest_rays, shadow_rays, samples, iterations;

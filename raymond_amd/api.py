@@ -115,9 +115,12 @@ class JobDesc(C.Structure):
                 ("max_depth", C.c_uint32), ("seed", C.c_uint32)]
 
 
+BATCH_SHORTEST_FIRST, BATCH_SAMPLE_MAJOR = 0, 1
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("rr_depth", C.c_uint32),
-                ("timing", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("timing", C.c_uint32), ("use_graph", C.c_uint32), ("order", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -492,8 +495,8 @@ class Batch:
         self.handle = C.c_void_p()
         _check(LIB.rm_batch_create(device.handle, arr, len(jobs), C.byref(self.handle)))
 
-    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0, use_graph=0):
-        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing, use_graph)
+    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0, use_graph=0, order=BATCH_SHORTEST_FIRST):
+        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing, use_graph, order)
         st = BatchStats()
         _check(LIB.rm_batch_render(self.handle, C.byref(cfg), C.byref(st)))
         return st

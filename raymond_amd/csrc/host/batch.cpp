@@ -15,8 +15,10 @@
 // world lights keep pointing at their envmaps).
 //
 // rm_batch_render walks the samples of all jobs in slices (one sample of one
-// job), ordered sample-major so that every job progresses from the start,
-// and splits them into bounce loops small enough for the loop's counters.
+// job), by default job after job in order of path count (shortest first: small
+// jobs complete early, and fewer jobs share the caches at a time), or
+// sample-major so that every job progresses from the start, and splits them
+// into bounce loops small enough for the loop's counters.
 // The renderer behind the batch runs those loops; the shading kernel counts
 // ended paths per job, from which the host records each job's completion.
 #include <algorithm>
@@ -179,6 +181,14 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
     for (uint32_t s = 0; s < max_spp; s++)
         for (uint32_t j = 0; j < n; j++)
             if (s < b->jobs[j].spp) b->slices.push_back(BatchSlice{0u, j, s, 0u});
+    // shortest job first: by path count, ties by job index
+    std::vector<uint32_t> by_size(n);
+    for (uint32_t j = 0; j < n; j++) by_size[j] = j;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](uint32_t x, uint32_t y) {
+        return b->job_paths[x] < b->job_paths[y];
+    });
+    for (uint32_t j : by_size)
+        for (uint32_t s = 0; s < b->jobs[j].spp; s++) b->slices_sjf.push_back(BatchSlice{0u, j, s, 0u});
     RM_HIP(hipStreamSynchronize(st));
 }
 
@@ -211,7 +221,9 @@ void batch_render(rm_batch* b, const rm_batch_config* cfg, rm_batch_stats* stats
     RM_HIP(hipMemsetAsync(b->d_done.ptr, 0, n * 4, st));
     b->job_ms.assign(n, -1.0);
 
-    const size_t slices = max_slices ? std::min(max_slices, b->slices.size()) : b->slices.size();
+    RM_REQUIRE(c.order == RM_BATCH_SHORTEST_FIRST || c.order == RM_BATCH_SAMPLE_MAJOR, "unknown batch order");
+    const std::vector<BatchSlice>& order = c.order == RM_BATCH_SHORTEST_FIRST ? b->slices_sjf : b->slices;
+    const size_t slices = max_slices ? std::min(max_slices, order.size()) : order.size();
     // bounce loops of at most this many paths: within the 32-bit counters and,
     // at a few paths per queue slot and iteration, within the iteration limit
     const uint64_t loop_limit = std::min<uint64_t>((uint64_t)capacity * 192, (1ull << 31) - 1);
@@ -228,10 +240,10 @@ void batch_render(rm_batch* b, const rm_batch_config* cfg, rm_batch_stats* stats
         uint64_t paths = 0;
         size_t last = first;
         while (last < slices) {
-            const JobInfo& ji = b->jobs[b->slices[last].job];
+            const JobInfo& ji = b->jobs[order[last].job];
             const uint64_t sz = (uint64_t)ji.width * ji.height;
             if (!loop.empty() && paths + sz > loop_limit) break;
-            BatchSlice sl = b->slices[last];
+            BatchSlice sl = order[last];
             sl.first = (uint32_t)paths;
             loop.push_back(sl);
             paths += sz;
@@ -359,6 +371,7 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
             c.queue_paths = queues[qi];
             c.material_sort = srt;
             c.timing = 0;
+            c.order = RM_BATCH_SAMPLE_MAJOR;  // the calibration prefix mixes all jobs
             rm_batch_stats s{};
             batch_render(b, &c, &s, prefix);
             const double rate = s.total_rays / (s.ms_total * 1e-3) / 1e6;
@@ -371,6 +384,7 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
         }
     if (count) *count = std::min(used, capacity);
     best->timing = base.timing;
+    best->order = base.order;
     // drop the calibration renders from the batch image and the job clocks
     renderer_reset(b->renderer.get());
     b->job_ms.assign(b->jobs.size(), -1.0);

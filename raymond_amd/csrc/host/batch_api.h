@@ -15,6 +15,7 @@ struct rm_batch {
     std::vector<rm::JobInfo> jobs;
     std::vector<uint64_t> job_paths;      // width x height x samples per job
     std::vector<rm::BatchSlice> slices;   // sample-major, firsts filled per bounce loop
+    std::vector<rm::BatchSlice> slices_sjf;  // shortest job first (fewest paths), samples in order
     std::vector<double> job_ms;           // completion time of each job in the last render
     rm::DevBuf d_jobs, d_lights, d_done, d_slices;
     uint32_t pixels = 0;

@@ -36,10 +36,10 @@ def solo(scene, w, h, spp, depth):
     return r.image(), st.total_rays
 
 
-@pytest.mark.parametrize("sort,graph", [(0, 0), (1, 0), (1, 1)])
-def test_batch_matches_solo_renders(device, jobs, sort, graph):
+@pytest.mark.parametrize("sort,graph,order", [(0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 0, 0), (1, 1, 0)])
+def test_batch_matches_solo_renders(device, jobs, sort, graph, order):
     b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
-    st = b.render(queue_paths=4096, material_sort=sort, use_graph=graph)
+    st = b.render(queue_paths=4096, material_sort=sort, use_graph=graph, order=order)
     total = 0
     for j, (s, w, h, spp, depth) in enumerate(jobs):
         ref, rays = solo(s, w, h, spp, depth)
@@ -80,3 +80,14 @@ def test_autotune_times_candidates_and_caches(device, jobs, tmp_path):
         assert st.total_rays > 0
     finally:
         del os.environ["RM_TUNE_CACHE"]
+
+
+def test_shortest_first_completes_small_jobs_first(device, jobs):
+    """With shortest-job-first slices the jobs complete in order of their
+    path counts (each job's last paths end before the next job's start)."""
+    b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
+    b.render(queue_paths=64, order=api.BATCH_SHORTEST_FIRST)
+    t = b.job_times()
+    paths = [w * h * spp for _, w, h, spp, _ in jobs]
+    order = np.argsort(paths, kind="stable")
+    assert np.all(np.diff(t[order]) >= 0.0)
