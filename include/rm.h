@@ -284,8 +284,8 @@ typedef struct rm_cache_desc {
     uint32_t seed;             /* parameter initialisation */
     float learning_rate;       /* Adam; default 1e-2 */
     uint32_t train_shift;      /* one camera path in 2^n trains; default 4 */
-    uint32_t batch_size;       /* samples per training step; default 16384 */
-    uint32_t steps_per_launch; /* training steps per sample per pixel rendered; default 4 */
+    uint32_t batch_size;       /* samples per training step; default 8192 */
+    uint32_t steps_per_launch; /* training steps per sample per pixel rendered; default 2 */
 } rm_cache_desc;
 
 typedef struct rm_cache_stats {
@@ -336,7 +336,7 @@ typedef struct rm_job_desc {
 } rm_job_desc;
 
 typedef struct rm_batch_config {
-    uint32_t queue_paths;    /* paths in flight; 0 = 2^21 */
+    uint32_t queue_paths;    /* paths in flight; 0 = 2^24 */
     uint32_t material_sort;  /* group each iteration's hits by material */
     uint32_t rr_depth;       /* Russian roulette from this depth (0 = from the first bounce) */
     uint32_t timing;         /* per-stage GPU times */

@@ -425,8 +425,8 @@ def _rows(a, k):
 class RadianceCache:
     """Learned radiance cache (rm_cache_*): hash-grid encoding + 48-64-64-3 MLP."""
 
-    def __init__(self, device, log2_table_size=19, seed=0, learning_rate=1e-2, train_shift=4, batch_size=16384,
-                 steps_per_launch=4):
+    def __init__(self, device, log2_table_size=19, seed=0, learning_rate=1e-2, train_shift=4, batch_size=8192,
+                 steps_per_launch=2):
         self.device = device
         self.desc = CacheDesc(log2_table_size, seed, learning_rate, train_shift, batch_size, steps_per_launch)
         self.handle = C.c_void_p()

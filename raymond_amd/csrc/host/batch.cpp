@@ -198,7 +198,7 @@ namespace rm {
 
 void batch_render(rm_batch* b, const rm_batch_config* cfg, rm_batch_stats* stats, size_t max_slices) {
     rm_batch_config c = cfg ? *cfg : rm_batch_config{};
-    if (!c.queue_paths) c.queue_paths = 1u << 21;
+    if (!c.queue_paths) c.queue_paths = 1u << 24;
     const size_t capacity = std::min<size_t>(c.queue_paths, 1u << 26);
     hipStream_t st = b->dev->stream;
     if (!b->renderer || b->renderer_capacity != capacity) {

@@ -112,7 +112,7 @@ def test_render_with_cache(device, cornell_scene):
     cs = c.stats()
     img = r.image()
     assert np.all(np.isfinite(img))
-    assert cs.queries > 0 and cs.train_records > 0 and cs.steps == 4 * 4
+    assert cs.queries > 0 and cs.train_records > 0 and cs.steps == 4 * 2
     # paths end at their first diffuse vertex, so far fewer rays are traced
     assert st.total_rays < 0.7 * sp.total_rays
     # frozen cache: no training vertices, no steps
