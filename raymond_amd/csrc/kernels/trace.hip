@@ -118,17 +118,36 @@ struct RayState {
         const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
         if (STATS) n_nodes++;
         const float px = __uint_as_float(q0.x), py = __uint_as_float(q0.y), pz = __uint_as_float(q0.z);
-        const float sx = exp_scale((uint8_t)(q0.w & 0xffu));
-        const float sy = exp_scale((uint8_t)((q0.w >> 8) & 0xffu));
-        const float sz = exp_scale((uint8_t)((q0.w >> 16) & 0xffu));
         const uint32_t imask = q0.w >> 24;
-        // near and far planes per axis for the lane's direction: the slab
-        // test's min/max pairs, chosen once per node (t is monotonic in the
-        // plane, so the values are those of min/max)
+        // Slab planes in ray space: t(q) = q * A + B with A = 2^(e-127) * inv
+        // (exact) and B = (p - o) * inv, one FMA per plane.  With u = 2^-24,
+        // fma(q, A, B) is within u (2 * 255 |A| + 4.01 |B|) of the plane's
+        // exact t, (q s + p - o) / d, so moving B outward by
+        // E = 16u (|B| + 255 |A|) leaves every near plane before and every
+        // far plane after the exact one, with a margin of at least
+        // 11u (|B| + 255 |A|) >= 11u |t| for the triangle test's own rounding
+        // (best_t keeps the relative slack on top).  A ray parallel to an
+        // axis has inv = +-1e30: the planes land at +-huge t and cull exactly
+        // when the origin is outside the slab; where B overflows, the NaN
+        // planes fall out of the max/min below (each drop widens [tn, tf]).
+        auto axis = [](float p, uint32_t ebits, float o, float inv, float& A, float& Bn, float& Bf) {
+            A = exp_scale((uint8_t)ebits) * inv;
+            const float B = (p - o) * inv;
+            const float E = 9.53674316e-7f * (fabsf(B) + 255.0f * fabsf(A));
+            Bn = B - E;
+            Bf = B + E;
+        };
+        float ax, bnx, bfx, ay, bny, bfy, az, bnz, bfz;
+        axis(px, q0.w & 0xffu, o.x, inv.x, ax, bnx, bfx);
+        axis(py, (q0.w >> 8) & 0xffu, o.y, inv.y, ay, bny, bfy);
+        axis(pz, (q0.w >> 16) & 0xffu, o.z, inv.z, az, bnz, bfz);
+        // near and far planes per axis for the lane's direction, chosen once
+        // per node (t decreases in q when inv < 0, so the high bytes are near)
         const bool fx = inv.x < 0.0f, fy = inv.y < 0.0f, fz = inv.z < 0.0f;
         const uint32_t nx0 = fx ? q3.z : q2.x, nx1 = fx ? q3.w : q2.y, xx0 = fx ? q2.x : q3.z, xx1 = fx ? q2.y : q3.w;
         const uint32_t ny0 = fy ? q4.x : q2.z, ny1 = fy ? q4.y : q2.w, xy0 = fy ? q2.z : q4.x, xy1 = fy ? q2.w : q4.y;
         const uint32_t nz0 = fz ? q4.z : q3.x, nz1 = fz ? q4.w : q3.y, xz0 = fz ? q3.x : q4.z, xz1 = fz ? q3.y : q4.w;
+        const float t_far = best_t * kBoxSlack;
         uint32_t ihits = 0, thits = 0;
         float near_t = kInf;
         int near_k = 0;
@@ -136,14 +155,14 @@ struct RayState {
         for (int c = 0; c < 8; c++) {
             const uint32_t meta = byte_of(q1.z, q1.w, c);
             if (meta == 0) continue;
-            const float tnx = (decode_q(px, (uint8_t)byte_of(nx0, nx1, c), sx) - o.x) * inv.x;
-            const float tfx = (decode_q(px, (uint8_t)byte_of(xx0, xx1, c), sx) - o.x) * inv.x;
-            const float tny = (decode_q(py, (uint8_t)byte_of(ny0, ny1, c), sy) - o.y) * inv.y;
-            const float tfy = (decode_q(py, (uint8_t)byte_of(xy0, xy1, c), sy) - o.y) * inv.y;
-            const float tnz = (decode_q(pz, (uint8_t)byte_of(nz0, nz1, c), sz) - o.z) * inv.z;
-            const float tfz = (decode_q(pz, (uint8_t)byte_of(xz0, xz1, c), sz) - o.z) * inv.z;
+            const float tnx = __builtin_fmaf((float)byte_of(nx0, nx1, c), ax, bnx);
+            const float tfx = __builtin_fmaf((float)byte_of(xx0, xx1, c), ax, bfx);
+            const float tny = __builtin_fmaf((float)byte_of(ny0, ny1, c), ay, bny);
+            const float tfy = __builtin_fmaf((float)byte_of(xy0, xy1, c), ay, bfy);
+            const float tnz = __builtin_fmaf((float)byte_of(nz0, nz1, c), az, bnz);
+            const float tfz = __builtin_fmaf((float)byte_of(xz0, xz1, c), az, bfz);
             const float tn = max_(max_(tnx, tny), max_(tnz, tmin));
-            const float tf = min_(min_(tfx, tfy), min_(tfz, best_t)) * kBoxSlack;
+            const float tf = min_(min_(tfx, tfy), min_(tfz, t_far));
             if (tn <= tf) {
                 if ((imask >> c) & 1u) {
                     ihits |= 1u << (c ^ oct);
