@@ -148,33 +148,37 @@ struct RayState {
         const uint32_t ny0 = fy ? q4.x : q2.z, ny1 = fy ? q4.y : q2.w, xy0 = fy ? q2.z : q4.x, xy1 = fy ? q2.w : q4.y;
         const uint32_t nz0 = fz ? q4.z : q3.x, nz1 = fz ? q4.w : q3.y, xz0 = fz ? q3.x : q4.z, xz1 = fz ? q3.y : q4.w;
         const float t_far = best_t * kBoxSlack;
-        uint32_t ihits = 0, thits = 0;
-        float near_t = kInf;
-        int near_k = 0;
+        // Branch-free child loop.  A hit child ORs (meta >> 5) << (meta & 31)
+        // into acc: a leaf's triangle span lands in bits 0-23, an internal
+        // child in slot c sets bit 24 + c.  fmaxf/fminf (v_max3/v_min3) drop
+        // NaN operands, which only widens [tn, tf].  The nearest internal
+        // child is the least key, tn's bits with the slot in the low 3 bits
+        // (for tn >= 0 the bits order like the values; the order steers the
+        // visit only, never the result).
+        uint32_t acc = 0, near_key = 0xffffffffu;
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const uint32_t meta = byte_of(q1.z, q1.w, c);
-            if (meta == 0) continue;
             const float tnx = __builtin_fmaf((float)byte_of(nx0, nx1, c), ax, bnx);
             const float tfx = __builtin_fmaf((float)byte_of(xx0, xx1, c), ax, bfx);
             const float tny = __builtin_fmaf((float)byte_of(ny0, ny1, c), ay, bny);
             const float tfy = __builtin_fmaf((float)byte_of(xy0, xy1, c), ay, bfy);
             const float tnz = __builtin_fmaf((float)byte_of(nz0, nz1, c), az, bnz);
             const float tfz = __builtin_fmaf((float)byte_of(xz0, xz1, c), az, bfz);
-            const float tn = max_(max_(tnx, tny), max_(tnz, tmin));
-            const float tf = min_(min_(tfx, tfy), min_(tfz, t_far));
-            if (tn <= tf) {
-                if ((imask >> c) & 1u) {
-                    ihits |= 1u << (c ^ oct);
-                    if (tn < near_t) {
-                        near_t = tn;
-                        near_k = c ^ oct;
-                    }
-                } else {
-                    thits |= (meta >> 5) << (meta & 31u);
-                }
-            }
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, t_far));
+            const bool hit = tn <= tf && meta != 0u;
+            acc |= hit ? (meta >> 5) << (meta & 31u) : 0u;
+            const uint32_t key = (__float_as_uint(tn) & ~7u) | (uint32_t)c;
+            near_key = hit && ((imask >> c) & 1u) ? min(near_key, key) : near_key;
         }
+        const uint32_t thits = acc & 0xffffffu;
+        // internal hits from slot order (bit c) to visit order (bit c ^ oct)
+        uint32_t ihits = acc >> 24;
+        ihits = (oct & 1) ? ((ihits & 0x55u) << 1) | ((ihits >> 1) & 0x55u) : ihits;
+        ihits = (oct & 2) ? ((ihits & 0x33u) << 2) | ((ihits >> 2) & 0x33u) : ihits;
+        ihits = (oct & 4) ? ((ihits & 0x0fu) << 4) | (ihits >> 4) : ihits;
+        const int near_k = (int)(near_key & 7u) ^ oct;
         cur = make_uint2(q1.x, (ihits << 24) | imask);
         // descend into the nearest child first (closest hit: best_t
         // shrinks sooner; any hit: near occluders are found sooner); the
