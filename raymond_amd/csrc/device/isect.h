@@ -8,9 +8,11 @@
 //     shear to the dominant axis, edge functions in float with a double
 //     re-evaluation when any is exactly zero, so no ray leaks between
 //     triangles sharing an edge;
-//   * boxes: slab test made conservative by scaling the exit distance by
-//     1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013), so a BVH
-//     can never cull a triangle that the exhaustive test would hit.
+//   * boxes (the oracle's BVH2): slab test made conservative by scaling the
+//     exit distance by 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT
+//     2013), so a BVH can never cull a triangle that the exhaustive test
+//     would hit.  The GPU traversal tests its quantised child boxes with its
+//     own bounded-error form of the slab test and uses kBoxSlack on best_t.
 // Closest-hit ties (equal t) go to the smaller (instance, primitive) pair, so
 // the result does not depend on traversal order.
 #pragma once
@@ -39,10 +41,11 @@ RM_HD RayShear make_shear(vec3 d) {
     return s;
 }
 
-// Returns true when the triangle is hit at t with tmin < t < tmax.
-// b0, b1: barycentric weights of v0 and v1 (v2 gets 1 - b0 - b1).
-RM_HD bool intersect_triangle(vec3 o, const RayShear& s, vec3 v0, vec3 v1, vec3 v2, float tmin, float tmax,
-                              float& t, float& b0, float& b1) {
+// Returns true when the triangle is hit at t with tmin < t < tmax, with the
+// scaled barycentrics U, V and their sum det: v0 and v1 weigh U / det and
+// V / det.  Traversal keeps these and divides once, for the closest hit.
+RM_HD bool intersect_triangle_uvd(vec3 o, const RayShear& s, vec3 v0, vec3 v1, vec3 v2, float tmin, float tmax,
+                                  float& t, float& u_out, float& v_out, float& det_out) {
     const vec3 A = v0 - o, B = v1 - o, C = v2 - o;
     const float Az_ = comp(A, s.kz), Bz_ = comp(B, s.kz), Cz_ = comp(C, s.kz);
     const float Ax = comp(A, s.kx) - s.sx * Az_;
@@ -68,6 +71,18 @@ RM_HD bool intersect_triangle(vec3 o, const RayShear& s, vec3 v0, vec3 v1, vec3 
     const float tt = T / det;
     if (!(tt > tmin && tt < tmax)) return false;
     t = tt;
+    u_out = U;
+    v_out = V;
+    det_out = det;
+    return true;
+}
+
+// Returns true when the triangle is hit at t with tmin < t < tmax.
+// b0, b1: barycentric weights of v0 and v1 (v2 gets 1 - b0 - b1).
+RM_HD bool intersect_triangle(vec3 o, const RayShear& s, vec3 v0, vec3 v1, vec3 v2, float tmin, float tmax,
+                              float& t, float& b0, float& b1) {
+    float U, V, det;
+    if (!intersect_triangle_uvd(o, s, v0, v1, v2, tmin, tmax, t, U, V, det)) return false;
     b0 = U / det;
     b1 = V / det;
     return true;

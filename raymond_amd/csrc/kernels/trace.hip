@@ -27,21 +27,40 @@
 
 namespace rm {
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+// bytes 2k, 2k+1 of w as the halves 1024 + byte (exact)
+__device__ inline half2v byte_pair(uint32_t w, int k) {
+    return __builtin_bit_cast(half2v, __builtin_amdgcn_perm(0x64646464u, w, k ? 0x04030402u : 0x04010400u));
+}
+__device__ inline float plane_q(const half2v (&h)[4], int c) { return (float)h[c >> 1][c & 1]; }
 __device__ inline uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
     const uint32_t w = c < 4 ? lo : hi;
     return (w >> (8 * (c & 3))) & 0xffu;
 }
 
+// Stack entries as 64-bit words behind pointers typed by address space, so
+// that the LDS rows and the spill rows stay ds_* and global_* accesses (a
+// generic pointer chosen between the two would make every stack access a
+// flat access).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef __attribute__((address_space(1))) unsigned long long global_u64;
+
+__device__ inline unsigned long long pack_entry(uint2 v) { return ((unsigned long long)v.y << 32) | v.x; }
+__device__ inline uint2 unpack_entry(unsigned long long w) { return make_uint2((uint32_t)w, (uint32_t)(w >> 32)); }
+
 struct StackRef {
-    uint2* lds;      // column base: lds[sp * 64]
-    uint2* spill;    // spill[(sp - kLdsStack) * stride]
+    lds_u64* lds;       // column base: lds[sp * 64]
+    global_u64* spill;  // spill[(sp - kLdsStack) * stride]
     uint32_t stride;
     __device__ void store(int sp, uint2 v) const {
-        if (sp < kLdsStack) lds[sp * kTraceBlock] = v;
-        else spill[(size_t)(sp - kLdsStack) * stride] = v;
+        if (sp < kLdsStack) lds[sp * kTraceBlock] = pack_entry(v);
+        else spill[(size_t)(sp - kLdsStack) * stride] = pack_entry(v);
     }
     __device__ uint2 load(int sp) const {
-        return sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(size_t)(sp - kLdsStack) * stride];
+        if (sp < kLdsStack) return unpack_entry(lds[sp * kTraceBlock]);
+        // a load of another kind, which the compiler cannot merge with the
+        // LDS load into one flat load
+        return unpack_entry(__builtin_nontemporal_load(&spill[(size_t)(sp - kLdsStack) * stride]));
     }
 };
 
@@ -53,7 +72,7 @@ struct RayState {
     int oct;
     float tmin, tmax, best_t;
     uint32_t best_inst, best_prim;
-    float bb0, bb1;
+    float bu, bv, bdet;  // closest hit's scaled barycentrics and their sum
     bool found;
     uint2 cur;  // node group: child base, (remaining child hits << 24) | internal mask
     uint2 tg;   // postponed leaf triangles: base, mask
@@ -73,8 +92,9 @@ struct RayState {
         best_t = t1;
         best_inst = 0xffffffffu;
         best_prim = 0xffffffffu;
-        bb0 = 0.0f;
-        bb1 = 0.0f;
+        bu = 0.0f;
+        bv = 0.0f;
+        bdet = 1.0f;
         found = false;
         tg = make_uint2(0u, 0u);
         n_nodes = 0;
@@ -120,22 +140,25 @@ struct RayState {
         const float px = __uint_as_float(q0.x), py = __uint_as_float(q0.y), pz = __uint_as_float(q0.z);
         const uint32_t imask = q0.w >> 24;
         // Slab planes in ray space: t(q) = q * A + B with A = 2^(e-127) * inv
-        // (exact) and B = (p - o) * inv, one FMA per plane.  With u = 2^-24,
-        // fma(q, A, B) is within u (2 * 255 |A| + 4.01 |B|) of the plane's
-        // exact t, (q s + p - o) / d, so moving B outward by
-        // E = 16u (|B| + 255 |A|) leaves every near plane before and every
-        // far plane after the exact one, with a margin of at least
-        // 11u (|B| + 255 |A|) >= 11u |t| for the triangle test's own rounding
-        // (best_t keeps the relative slack on top).  A ray parallel to an
-        // axis has inv = +-1e30: the planes land at +-huge t and cull exactly
-        // when the origin is outside the slab; where B overflows, the NaN
-        // planes fall out of the max/min below (each drop widens [tn, tf]).
+        // (exact) and B = (p - o) * inv, one FMA per plane.  The byte q
+        // enters as the half 1024 + q (two per v_perm, exact), which
+        // v_fma_mix_f32 multiplies in f32, so the offset becomes
+        // B' = (B -+ E) - 1024 A.  With u = 2^-24 the computed plane is within
+        // u (1534 |A| + 6 |B|) of its exact t, (q s + p - o) / d, to first
+        // order, so moving B outward by E = 16u (|B| + 255 |A|) leaves every
+        // near plane before and every far plane after the exact one, with a
+        // margin of nearly 10u (|B| + 255 |A|) >= 10u |t| for the triangle
+        // test's own rounding (best_t keeps the relative slack on top).  A
+        // ray parallel to an axis has inv = +-1e30: the planes land at +-huge
+        // t and cull exactly when the origin is outside the slab; where B
+        // overflows, the NaN planes fall out of the max/min below (each drop
+        // widens [tn, tf]).
         auto axis = [](float p, uint32_t ebits, float o, float inv, float& A, float& Bn, float& Bf) {
             A = exp_scale((uint8_t)ebits) * inv;
             const float B = (p - o) * inv;
             const float E = 9.53674316e-7f * (fabsf(B) + 255.0f * fabsf(A));
-            Bn = B - E;
-            Bf = B + E;
+            Bn = __builtin_fmaf(-1024.0f, A, B - E);
+            Bf = __builtin_fmaf(-1024.0f, A, B + E);
         };
         float ax, bnx, bfx, ay, bny, bfy, az, bnz, bfz;
         axis(px, q0.w & 0xffu, o.x, inv.x, ax, bnx, bfx);
@@ -148,6 +171,12 @@ struct RayState {
         const uint32_t ny0 = fy ? q4.x : q2.z, ny1 = fy ? q4.y : q2.w, xy0 = fy ? q2.z : q4.x, xy1 = fy ? q2.w : q4.y;
         const uint32_t nz0 = fz ? q4.z : q3.x, nz1 = fz ? q4.w : q3.y, xz0 = fz ? q3.x : q4.z, xz1 = fz ? q3.y : q4.w;
         const float t_far = best_t * kBoxSlack;
+        const half2v hnx[4] = {byte_pair(nx0, 0), byte_pair(nx0, 1), byte_pair(nx1, 0), byte_pair(nx1, 1)};
+        const half2v hxx[4] = {byte_pair(xx0, 0), byte_pair(xx0, 1), byte_pair(xx1, 0), byte_pair(xx1, 1)};
+        const half2v hny[4] = {byte_pair(ny0, 0), byte_pair(ny0, 1), byte_pair(ny1, 0), byte_pair(ny1, 1)};
+        const half2v hxy[4] = {byte_pair(xy0, 0), byte_pair(xy0, 1), byte_pair(xy1, 0), byte_pair(xy1, 1)};
+        const half2v hnz[4] = {byte_pair(nz0, 0), byte_pair(nz0, 1), byte_pair(nz1, 0), byte_pair(nz1, 1)};
+        const half2v hxz[4] = {byte_pair(xz0, 0), byte_pair(xz0, 1), byte_pair(xz1, 0), byte_pair(xz1, 1)};
         // Branch-free child loop.  A hit child ORs (meta >> 5) << (meta & 31)
         // into acc: a leaf's triangle span lands in bits 0-23, an internal
         // child in slot c sets bit 24 + c.  fmaxf/fminf (v_max3/v_min3) drop
@@ -159,12 +188,12 @@ struct RayState {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const uint32_t meta = byte_of(q1.z, q1.w, c);
-            const float tnx = __builtin_fmaf((float)byte_of(nx0, nx1, c), ax, bnx);
-            const float tfx = __builtin_fmaf((float)byte_of(xx0, xx1, c), ax, bfx);
-            const float tny = __builtin_fmaf((float)byte_of(ny0, ny1, c), ay, bny);
-            const float tfy = __builtin_fmaf((float)byte_of(xy0, xy1, c), ay, bfy);
-            const float tnz = __builtin_fmaf((float)byte_of(nz0, nz1, c), az, bnz);
-            const float tfz = __builtin_fmaf((float)byte_of(xz0, xz1, c), az, bfz);
+            const float tnx = __builtin_fmaf(plane_q(hnx, c), ax, bnx);
+            const float tfx = __builtin_fmaf(plane_q(hxx, c), ax, bfx);
+            const float tny = __builtin_fmaf(plane_q(hny, c), ay, bny);
+            const float tfy = __builtin_fmaf(plane_q(hxy, c), ay, bfy);
+            const float tnz = __builtin_fmaf(plane_q(hnz, c), az, bnz);
+            const float tfz = __builtin_fmaf(plane_q(hxz, c), az, bfz);
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
             const float tf = fminf(fminf(tfx, tfy), fminf(tfz, t_far));
             const bool hit = tn <= tf && meta != 0u;
@@ -204,14 +233,15 @@ struct RayState {
             const vec3 v0(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
             const vec3 v1(__uint_as_float(bq.x), __uint_as_float(bq.y), __uint_as_float(bq.z));
             const vec3 v2(__uint_as_float(cq.x), __uint_as_float(cq.y), __uint_as_float(cq.z));
-            float t, b0, b1;
-            if (intersect_triangle(o, sh, v0, v1, v2, tmin, tmax, t, b0, b1) &&
+            float t, u, v, det;
+            if (intersect_triangle_uvd(o, sh, v0, v1, v2, tmin, tmax, t, u, v, det) &&
                 hit_before(t, a.w, bq.w, best_t, best_inst, best_prim)) {
                 best_t = t;
                 best_inst = a.w;
                 best_prim = bq.w;
-                bb0 = b0;
-                bb1 = b1;
+                bu = u;
+                bv = v;
+                bdet = det;
                 found = true;
                 if (ANY) return true;
             }
@@ -314,7 +344,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
+    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock};
     const uint32_t* count_ptr = a.count_ptr;
     uint32_t* fetch = a.fetch;
     if (a.iter_dev) {
@@ -364,8 +394,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
             a.hits.t[i] = s.found ? s.best_t : -1.0f;
             a.hits.prim[i] = s.best_prim;
             a.hits.inst[i] = s.best_inst;
-            a.hits.b0[i] = s.bb0;
-            a.hits.b1[i] = s.bb1;
+            a.hits.b0[i] = s.bu / s.bdet;
+            a.hits.b1[i] = s.bv / s.bdet;
             if (STATS) {
                 sn += s.n_nodes;
                 st += s.n_tris;
@@ -451,7 +481,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    StackRef stk{lds + lane, a.spill + gtid, gridDim.x * kTraceBlock};
+    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock};
     const uint32_t* count_ptr = a.count_ptr;
     uint32_t* fetch = a.fetch;
     if (a.iter_dev) {
