@@ -86,6 +86,12 @@ typedef struct rm_instance_desc {
     float transform[16];    /* column-major 4x4 (columns c0..c3), object -> world */
     uint32_t visibility;    /* RM_RAY_* mask */
     uint32_t emissive;      /* non-zero: register as a shape light (EntityBuilder.swift:103-114) */
+    uint32_t has_bounds;    /* non-zero: bounds_min/bounds_max are the object-space bounds of the
+                               "Generated" texture coordinates (PerInstanceData.hpp:13-14, as the
+                               reference's PLY reader computes them, PLYReader.m:104-108); zero:
+                               the exact bounds of the instance's vertices */
+    float bounds_min[3];
+    float bounds_max[3];
 } rm_instance_desc;
 
 typedef struct rm_light_info {

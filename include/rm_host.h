@@ -44,6 +44,33 @@ int rmh_bvh8_check(const float* tris, uint32_t n, int threads, uint32_t* out_nod
 int rmh_jit_precompile(const char* source, size_t length, const char* options, int* out_hit, char* log,
                        size_t log_capacity);
 
+/* A mesh as the Blender exporter writes it (shapes.py:19-123): per vertex
+ * position, normal and uv; per face three vertex indices and the slot of the
+ * face's material in the shape's "materials" list. */
+typedef struct rmh_ply_mesh {
+    uint32_t num_vertices;
+    uint32_t num_faces;
+    float* vertices;        /* 3 per vertex */
+    float* normals;         /* 3 per vertex */
+    float* texcoords;       /* 2 per vertex */
+    uint32_t* indices;      /* 3 per face */
+    uint8_t* face_palette;  /* per face: index into the shape's material list */
+    float bounds_min[3];    /* as the reference's reader computes them (see below) */
+    float bounds_max[3];
+} rmh_ply_mesh;
+
+/* Reads an ASCII PLY file of the exporter's exact layout.  Replaces the
+ * reference's PLYReader (raymond/io/PLYReader.m:1-169) as ShapeBuilder uses it
+ * (host/scene/ShapeBuilder.swift:150-187, 260-283): the same header lines,
+ * numbers parsed as (float)digits * 10^-decimals in float arithmetic, and
+ * bounds whose maximum is only raised by a coordinate that is not a new
+ * minimum.  Returns 0 with the arrays malloc'd into *out (free with
+ * rmh_ply_free), or -1 and a malloc'd message (free with rmh_free) where the
+ * reference would assert: a header line that differs, a face that is not a
+ * triangle, a malformed number. */
+int rmh_ply_read(const char* path, rmh_ply_mesh* out, char** out_error);
+void rmh_ply_free(rmh_ply_mesh* mesh);
+
 #ifdef __cplusplus
 }
 #endif

@@ -561,6 +561,11 @@ or_scene* or_scene_create(const rm_scene_desc* d, const char* material_so, char*
             }
         if (id.vertex_count == 0)
             for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0.0f;
+        if (id.has_bounds)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = id.bounds_min[a];
+                hi[a] = id.bounds_max[a];
+            }
         pi.bounds_min = vec3(lo[0], lo[1], lo[2]);
         pi.bounds_size = vec3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
         pi.point_transform = mat4_of(id.transform);

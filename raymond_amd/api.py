@@ -30,7 +30,8 @@ class DeviceInfo(C.Structure):
 class InstanceDesc(C.Structure):
     _fields_ = [("vertex_offset", C.c_uint32), ("vertex_count", C.c_uint32), ("face_offset", C.c_uint32),
                 ("face_count", C.c_uint32), ("transform", C.c_float * 16), ("visibility", C.c_uint32),
-                ("emissive", C.c_uint32)]
+                ("emissive", C.c_uint32), ("has_bounds", C.c_uint32), ("bounds_min", C.c_float * 3),
+                ("bounds_max", C.c_float * 3)]
 
 
 class LightInfo(C.Structure):
@@ -191,6 +192,8 @@ EXPORTS = {
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "rmh_free": (None, [C.c_void_p]),
+    "rmh_ply_read": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "rmh_ply_free": (None, [C.c_void_p]),
     "rmh_jit_precompile": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "rmh_bvh8_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.c_char_p, C.c_size_t]),
@@ -232,6 +235,40 @@ def codegen(materials_json, noinline=False):
     out = C.cast(src, C.c_char_p).value.decode()
     LIB.rmh_free(src)
     return out
+
+
+class PlyMesh(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_faces", C.c_uint32), ("vertices", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("texcoords", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("face_palette", C.POINTER(C.c_uint8)),
+                ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3)]
+
+
+def read_ply(path):
+    """The exporter's ASCII PLY (rmh_ply_read; PLYReader.m:1-169): a dict of
+    vertices (n, 3), normals (n, 3), texcoords (n, 2), faces (m, 3),
+    face_palette (m,) and the reader's bounds_min / bounds_max."""
+    mesh = PlyMesh()
+    err = C.c_void_p()
+    if LIB.rmh_ply_read(os.fsencode(path), C.byref(mesh), C.byref(err)) != 0:
+        msg = C.cast(err, C.c_char_p).value.decode() if err.value else "PLY read failed"
+        LIB.rmh_free(err)
+        raise RmError(msg)
+    try:
+        nv, nf = mesh.num_vertices, mesh.num_faces
+
+        def take(ptr, n, k, dtype):
+            if n == 0:
+                return np.zeros((0, k), dtype)
+            return np.ctypeslib.as_array(ptr, shape=(n * k,)).reshape(n, k).astype(dtype, copy=True)
+
+        return dict(vertices=take(mesh.vertices, nv, 3, np.float32), normals=take(mesh.normals, nv, 3, np.float32),
+                    texcoords=take(mesh.texcoords, nv, 2, np.float32), faces=take(mesh.indices, nf, 3, np.uint32),
+                    face_palette=take(mesh.face_palette, nf, 1, np.uint8).reshape(-1),
+                    bounds_min=np.array(mesh.bounds_min[:], np.float32),
+                    bounds_max=np.array(mesh.bounds_max[:], np.float32))
+    finally:
+        LIB.rmh_ply_free(C.byref(mesh))
 
 
 def identifier(name):

@@ -115,6 +115,10 @@ void scene_create(rm_device* dev, const rm_scene_desc* d, rm_scene* s) {
             bmax = vmax(bmax, vec3(p[0], p[1], p[2]));
         }
         if (id.vertex_count == 0) bmin = bmax = vec3(0.0f);
+        if (id.has_bounds) {
+            bmin = vec3(id.bounds_min[0], id.bounds_min[1], id.bounds_min[2]);
+            bmax = vec3(id.bounds_max[0], id.bounds_max[1], id.bounds_max[2]);
+        }
         pi.bounds_min = bmin;
         pi.bounds_size = bmax - bmin;
         pi.point_transform = mat4_from(id.transform);

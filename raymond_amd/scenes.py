@@ -172,6 +172,9 @@ class Shape:
         if texcoords is None:
             texcoords = np.zeros((self.vertices.shape[0], 2), np.float32)
         self.texcoords = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+        # object-space bounds of the "Generated" coordinates; None: the exact
+        # bounds of the vertices (a loaded PLY brings its reader's bounds)
+        self.bounds = None
 
 
 class SceneDescription:
@@ -225,7 +228,7 @@ class SceneDescription:
             s = self.shapes[si]
             inst.append(dict(vertex_offset=vo[si], vertex_count=s.vertices.shape[0], face_offset=fo[si],
                              face_count=s.faces.shape[0], transform=col_major(m), visibility=vis,
-                             emissive=int(em)))
+                             emissive=int(em), bounds=s.bounds))
         return (np.ascontiguousarray(verts, np.float32), np.ascontiguousarray(norms, np.float32),
                 np.ascontiguousarray(uvs, np.float32), np.ascontiguousarray(faces, np.uint32),
                 np.ascontiguousarray(mats, np.uint16), inst)
@@ -248,8 +251,11 @@ class SceneDescription:
         sd.face_materials = mats.ctypes.data_as(C.POINTER(C.c_uint16))
         arr = (api.InstanceDesc * max(1, len(inst)))()
         for k, d in enumerate(inst):
+            b = d["bounds"]
             arr[k] = api.InstanceDesc(d["vertex_offset"], d["vertex_count"], d["face_offset"], d["face_count"],
-                                      (C.c_float * 16)(*d["transform"]), d["visibility"], d["emissive"])
+                                      (C.c_float * 16)(*d["transform"]), d["visibility"], d["emissive"],
+                                      int(b is not None), (C.c_float * 3)(*(b[0] if b is not None else (0, 0, 0))),
+                                      (C.c_float * 3)(*(b[1] if b is not None else (0, 0, 0))))
         self._keep.append(arr)
         sd.num_instances = len(inst)
         sd.instances = arr
