@@ -245,6 +245,9 @@ def main():
                          "graphs: 32 random node-graph materials; "
                          "cache: learned radiance cache against the plain tracer on the Cornell box; "
                          "batch: 64 uneven Cornell-box jobs in one bounce loop")
+    ap.add_argument("--scene", default="",
+                    help="render a scene file of the Blender exporter (JSON + PLY meshes) instead of a workload; "
+                         "its render resolution applies unless --width/--height are given")
     ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
     ap.add_argument("--cache-batch", type=int, default=8192, help="cache workload: samples per training step")
@@ -277,6 +280,16 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
     W, H = args.width, args.height
+    loaded = None
+    if args.scene:
+        import raymond_amd.rayjay as rayjay
+        t_load = time.time()
+        loaded = rayjay.load_scene(args.scene)
+        load_s = time.time() - t_load
+        if loaded.resolution and "--width" not in sys.argv and "--height" not in sys.argv:
+            W, H = loaded.resolution
+        print("loaded %s in %.2fs: %d shapes, %d triangles" % (args.scene, load_s, len(loaded.shapes),
+                                                             loaded.num_triangles), file=sys.stderr, flush=True)
     graphs = args.workload == "graphs"
     cornell = args.workload == "cornell"
     if cornell and (W, H) == (1920, 1080):
@@ -290,7 +303,9 @@ def main():
     # scenes the reference's way, from the first bounce
     rr_depth = 3 if cornell else 0
     t0 = time.time()
-    if graphs:
+    if loaded is not None:
+        desc = loaded
+    elif graphs:
         desc = scenes.shader_graph_scene(W, H)
     elif cornell:
         desc = scenes.cornell_box(W, H)
@@ -387,7 +402,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": workload_name(graphs, W, H, D, sort, cornell),
+            "workload": workload_name(graphs, W, H, D, sort, cornell) if loaded is None else
+            "exporter scene %s: %d triangles, %dx%d, depth %d, plain tracer, 1 spp per step" %
+            (os.path.basename(args.scene), int(sst.triangles), W, H, D),
             "width": W, "height": H, "max_depth": D, "queue_frames": spl, "triangles": int(sst.triangles),
             "iterations": int(totals["iters"]), "material_sort": bool(sort),
         },

@@ -42,6 +42,7 @@ Principled BSDF whose emission colour and strength are linked or non-zero.
 import json
 import math
 import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -169,19 +170,24 @@ def load_scene(path, envmap_resolution=2048):
         mats.get("light", light["material"])
 
     # shapes, in the order the entities name them (EntityBuilder.swift:44-52)
-    shape_index, shape_emissive = {}, []
+    order, named = [], set()
     for ename, ent in entities.items():
         sname = ent.get("shape")
-        if sname in shape_index:
-            continue
         if sname not in shapes:
             raise SceneFormatError("entity '%s' names unknown shape '%s'" % (ename, sname))
+        if sname not in named:
+            named.add(sname)
+            if shapes[sname].get("type") != "ply":
+                raise SceneFormatError("shape '%s': unsupported type '%s'" % (sname, shapes[sname].get("type")))
+            order.append(sname)
+    # the meshes parse in parallel (the reader releases the GIL)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1, max(1, len(order)))) as pool:
+        meshes = list(pool.map(lambda n: api.read_ply(os.path.join(base, shapes[n]["filepath"])), order))
+    shape_index, shape_emissive = {}, []
+    for sname, mesh in zip(order, meshes):
         sh = shapes[sname]
-        if sh.get("type") != "ply":
-            raise SceneFormatError("shape '%s': unsupported type '%s'" % (sname, sh.get("type")))
         palette = np.array([mats.get("surface", m) for m in sh.get("materials", [])], np.uint16)
         emissive = any(has_surface_emission(materials[m]) for m in sh.get("materials", []))
-        mesh = api.read_ply(os.path.join(base, sh["filepath"]))
         if mesh["face_palette"].size and int(mesh["face_palette"].max()) >= palette.size:
             raise SceneFormatError("shape '%s': face material slot beyond its %d materials" % (sname, palette.size))
         shape = scenes.Shape(mesh["vertices"], mesh["normals"], mesh["faces"],
