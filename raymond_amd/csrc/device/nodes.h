@@ -497,6 +497,402 @@ struct AddShader {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Further converters, textures and BSDFs (device/nodes/nodes.hpp).  Each
+// follows the reference's node, simplifications included; field names are
+// the codegen's identifiers of the exporter's socket names.
+
+// (device/nodes/nodes.hpp:15-46); flags as 0/1 floats
+struct LightPath {
+    float isCameraRay = 0.0f, isReflectionRay = 0.0f, isTransmissionRay = 0.0f, isShadowRay = 0.0f;
+    float isDiffuseRay = 0.0f, isGlossyRay = 0.0f, isSingularRay = 0.0f, rayLength = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        auto flag = [&](uint32_t f) { return (s.ray_flags & f) ? 1.0f : 0.0f; };
+        isCameraRay = flag(RayCamera);
+        isReflectionRay = flag(RayReflection);
+        isTransmissionRay = flag(RayTransmission);
+        isShadowRay = flag(RayShadow);
+        isDiffuseRay = flag(RayDiffuse);
+        isGlossyRay = flag(RayGlossy);
+        isSingularRay = flag(RaySingular);
+        rayLength = s.distance;
+        if (isSingularRay != 0.0f) isGlossyRay = 1.0f;
+    }
+};
+
+struct kVectorMath {
+    enum Operation {
+        OPERATION_ADD, OPERATION_SUBTRACT, OPERATION_MULTIPLY, OPERATION_MULTIPLY_ADD, OPERATION_NORMALIZE,
+        OPERATION_SCALE, OPERATION_MINIMUM, OPERATION_LENGTH, OPERATION_DOT_PRODUCT
+    };
+};
+
+// (device/nodes/nodes.hpp:151-199); the dot product lands in `vector`
+template <int Operation>
+struct VectorMath {
+    float scale = 0.0f, value = 0.0f;
+    vec3 vector, vector_001, vector_002;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        switch (Operation) {
+        case kVectorMath::OPERATION_ADD: vector = vector + vector_001; break;
+        case kVectorMath::OPERATION_SUBTRACT: vector = vector - vector_001; break;
+        case kVectorMath::OPERATION_MULTIPLY: vector = vector * vector_001; break;
+        case kVectorMath::OPERATION_MULTIPLY_ADD: vector = vector * vector_001 + vector_002; break;
+        case kVectorMath::OPERATION_NORMALIZE: vector = normalize(vector); break;
+        case kVectorMath::OPERATION_SCALE: vector = vector * scale; break;
+        case kVectorMath::OPERATION_MINIMUM: vector = vmin(vector, vector_001); break;
+        case kVectorMath::OPERATION_LENGTH: value = length(vector); break;
+        case kVectorMath::OPERATION_DOT_PRODUCT: vector = vec3(dot(vector, vector_001)); break;
+        }
+    }
+};
+
+// floor to int, saturated to the int range so that the GPU and the host agree
+RM_HD int floor_to_int(float x) {
+    const float f = floor_(x);
+    if (!(f >= -2147483648.0f)) return (int)0x80000000;
+    if (f >= 2147483648.0f) return 0x7fffffff;
+    return (int)f;
+}
+
+// (device/nodes/nodes.hpp:247-264)
+struct TexChecker {
+    float scale = 0.0f;
+    vec4 color1, color2;
+    vec3 vector;
+    float fac = 0.0f;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 p = (vector * scale + 0.000001f) * 0.999999f;
+        const bool which = ((floor_to_int(p.x) ^ floor_to_int(p.y) ^ floor_to_int(p.z)) & 1) != 0;
+        color = which ? color1 : color2;
+        fac = which ? 1.0f : 0.0f;
+    }
+};
+
+// (device/nodes/nodes.hpp:266-284)
+struct SeparateVector {
+    vec3 vector;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        x = vector.x;
+        y = vector.y;
+        z = vector.z;
+    }
+};
+struct CombineVector {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    vec3 vector;
+    RM_HD void compute(const SceneView&, ShadingContext&) { vector = vec3(x, y, z); }
+};
+
+// (device/nodes/nodes.hpp:304-344): float data, linear interpolation
+template <bool Clamp>
+struct MapRange {
+    float fromMin = 0.0f, fromMax = 0.0f, toMin = 0.0f, toMax = 0.0f, steps = 0.0f;
+    vec3 steps_float3, from_min_float3, from_max_float3, to_min_float3, to_max_float3;
+    float value = 0.0f;
+    vec3 vector;
+    float result = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        float v = (value - fromMin) / (fromMax - fromMin);
+        if (Clamp) v = saturate(v);
+        result = lerpf(toMin, toMax, v);
+    }
+};
+
+// XYZ Euler rotation, the matrix of node_mapping.osl (device/utils/math.hpp:43-67)
+RM_HD vec3 euler_rotate(vec3 e, vec3 v) {
+    float cx, cy, cz;
+    const float sx = sincos_(e.x, cx);
+    const float sy = sincos_(e.y, cy);
+    const float sz = sincos_(e.z, cz);
+    const vec3 c0(cy * cz, cy * sz, -sy);
+    const vec3 c1(sy * sx * cz - cx * sz, sy * sx * sz + cx * cz, cy * sx);
+    const vec3 c2(sy * cx * cz + sx * sz, sy * cx * sz - sx * cz, cy * cx);
+    return (c0 * v.x + c1 * v.y) + c2 * v.z;
+}
+
+// (device/nodes/nodes.hpp:346-364): the point mapping (the exporter writes
+// the node's type, "MAPPING", not its vector type)
+struct Mapping {
+    vec3 scale, rotation, location, vector;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        vector = euler_rotate(rotation, vector * scale) + location;
+    }
+};
+
+struct kSeparateColor { enum Mode { MODE_RGB, MODE_HSV }; };
+struct kCombineColor { enum Mode { MODE_RGB, MODE_HSV }; };
+
+// (device/nodes/nodes.hpp:1195-1259); combining HSV passes the values
+// through as the reference does
+template <int Mode>
+struct SeparateColor {
+    vec4 color;
+    float red = 0.0f, green = 0.0f, blue = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 c = Mode == kSeparateColor::MODE_HSV ? rgb2hsv(color.xyz()) : color.xyz();
+        red = c.x;
+        green = c.y;
+        blue = c.z;
+    }
+};
+template <int Mode>
+struct CombineColor {
+    float red = 0.0f, green = 0.0f, blue = 0.0f;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) { color = vec4(red, green, blue, 1.0f); }
+};
+
+// (device/nodes/nodes.hpp:1261-1277)
+struct HueSaturation {
+    vec4 color;
+    float fac = 0.0f, hue = 0.0f, saturation = 0.0f, value = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        vec3 hsv = rgb2hsv(color.xyz());
+        hsv.x = fmod_(hsv.x + hue + 0.5f, 1.0f);
+        hsv.y = saturate(hsv.y * saturation);
+        hsv.z *= value;
+        const vec3 result = vmax(hsv2rgb(hsv), vec3(0.0f));
+        color = vec4(lerp(color.xyz(), result, fac), color.w);
+    }
+};
+
+// (device/nodes/nodes.hpp:1279-1290); alpha goes through the same map
+struct BrightnessContrast {
+    float bright = 0.0f, contrast = 0.0f;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const float a = 1.0f + contrast;
+        const float b = bright - contrast / 2.0f;
+        color = vmax4(a * color + b, vec4(0.0f));
+    }
+};
+
+// (device/nodes/nodes.hpp:1292-1302)
+struct Gamma {
+    vec4 color;
+    float gamma = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        if (gamma == 0.0f) {
+            color.x = color.y = color.z = 1.0f;
+        } else {
+            if (color.x > 0.0f) color.x = pow_(color.x, gamma);
+            if (color.y > 0.0f) color.y = pow_(color.y, gamma);
+            if (color.z > 0.0f) color.z = pow_(color.z, gamma);
+        }
+    }
+};
+
+// (device/nodes/nodes.hpp:1387-1394)
+struct ColorInvert {
+    vec4 color;
+    float fac = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 c = color.xyz();
+        const vec3 r = c - fac * (2.0f * c - 1.0f);
+        color = vec4(r, color.w);
+    }
+};
+
+// (device/nodes/nodes.hpp:1726-1734)
+struct RGBToBW {
+    vec4 color;
+    float val = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) { val = luminance(color.xyz()); }
+};
+
+// (device/nodes/nodes.hpp:1112-1126)
+struct Fresnel {
+    float ior = 0.0f;
+    vec3 normal;
+    float fac = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        const float cos_i = dot(s.wo, normal);
+        float eta = max_(ior, 1e-5f);
+        if (cos_i < 0.0f) eta = 1.0f / eta;
+        fac = fresnel_dielectric_cos(cos_i, eta);
+    }
+};
+
+// (device/nodes/nodes.hpp:1689-1714)
+struct LayerWeight {
+    float blend = 0.0f;
+    vec3 normal;
+    float fresnel = 0.0f, facing = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        const float cos_i = dot(s.wo, normal);
+        float eta = max_(1.0f - blend, 1e-5f);
+        eta = cos_i < 0.0f ? eta : 1.0f / eta;
+        fresnel = fresnel_dielectric_cos(cos_i, eta);
+        facing = abs_(cos_i);
+        if (blend != 0.5f) {
+            float b = min_(max_(blend, 0.0f), 1.0f - 1e-5f);
+            b = b < 0.5f ? 2.0f * b : 0.5f / (1.0f - b);
+            facing = pow_(facing, b);
+        }
+        facing = 1.0f - facing;
+    }
+};
+
+struct kMix { enum FactorMode { FACTOR_MODE_UNIFORM, FACTOR_MODE_NON_UNIFORM }; };
+
+// (device/nodes/nodes.hpp:1933-1974): every data type is computed; colours
+// always blend linearly
+template <bool ClampFactor, bool ClampResult, int FactorMode>
+struct Mix {
+    vec4 a_color, b_color, result_color;
+    float a_float = 0.0f, b_float = 0.0f, result_float = 0.0f;
+    vec3 a_vector, b_vector, result_vector;
+    float factor_float = 0.0f;
+    vec3 factor_vector;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        if (ClampFactor) {
+            factor_float = saturate(factor_float);
+            factor_vector = saturate(factor_vector);
+        }
+        if (FactorMode == kMix::FACTOR_MODE_UNIFORM) result_vector = lerp(a_vector, b_vector, factor_float);
+        else result_vector = a_vector + (b_vector - a_vector) * factor_vector;
+        result_float = lerpf(a_float, b_float, factor_float);
+        result_color = lerp(a_color, b_color, factor_float);
+        if (ClampResult) result_color = saturate(result_color);
+    }
+};
+
+struct kTexGradient {
+    enum Type { TYPE_LINEAR, TYPE_SPHERICAL, TYPE_EASING, TYPE_QUADRATIC, TYPE_QUADRATIC_SPHERE };
+};
+
+// (device/nodes/nodes.hpp:366-405): the easing and quadratic types use the
+// linear ramp, as in the reference
+template <int Type>
+struct TexGradient {
+    vec3 vector;
+    vec4 color;
+    float fac = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const float v = Type == kTexGradient::TYPE_SPHERICAL ? saturate(length(vector)) : saturate(vector.x);
+        color = vec4(v);
+        fac = color.x;
+    }
+};
+
+// Glossy and anisotropic: one GGX specular lobe (device/nodes/nodes.hpp:1534-1567, 1746-1770)
+struct BsdfGlossy {
+    vec4 color;
+    vec3 normal;
+    float roughness = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const float alpha = square(max_(roughness, 1e-4f));
+        bsdf.specular.alpha_x = alpha;
+        bsdf.specular.alpha_y = alpha;
+        bsdf.specular.cspec0 = color.xyz();
+        bsdf.specular.ior = 1.45f;
+        bsdf.specular.weight = 1.0f;
+        bsdf.lobe_prob[1] = 1.0f;
+        bsdf.normal = normal;
+    }
+};
+struct BsdfAnisotropic {
+    float anisotropy = 0.0f;
+    vec4 color;
+    vec3 normal;
+    float rotation = 0.0f;
+    vec3 tangent;
+    float roughness = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView& v, ShadingContext& s) {
+        BsdfGlossy g;
+        g.color = color;
+        g.normal = normal;
+        g.roughness = roughness;
+        g.compute(v, s);
+        bsdf = g.bsdf;
+    }
+};
+
+// Glass and refraction: the transmission lobe (device/nodes/nodes.hpp:1497-1532, 1772-1795)
+struct BsdfGlass {
+    vec4 color;
+    float ior = 0.0f;
+    vec3 normal;
+    float roughness = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const float alpha = square(max_(roughness, 1e-4f));
+        bsdf.transmission.reflection_alpha = alpha;
+        bsdf.transmission.transmission_alpha = alpha;
+        bsdf.transmission.base_color = color.xyz();
+        bsdf.transmission.cspec0 = color.xyz();
+        bsdf.transmission.ior = ior;
+        bsdf.transmission.weight = 1.0f;
+        bsdf.lobe_prob[2] = 1.0f;
+        bsdf.normal = normal;
+    }
+};
+struct BsdfRefraction {
+    vec4 color;
+    float ior = 0.0f;
+    vec3 normal;
+    float roughness = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const float alpha = square(max_(roughness, 1e-4f));
+        bsdf.transmission.reflection_alpha = alpha;
+        bsdf.transmission.transmission_alpha = alpha;
+        bsdf.transmission.base_color = color.xyz();
+        bsdf.transmission.cspec0 = vec3(0.0f);
+        bsdf.transmission.ior = ior;
+        bsdf.transmission.weight = 1.0f;
+        bsdf.transmission.only_refract = true;
+        bsdf.lobe_prob[2] = 1.0f;
+        bsdf.normal = normal;
+    }
+};
+
+// (device/nodes/nodes.hpp:1797-1807)
+struct BsdfTransparent {
+    vec4 color;
+    float weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        bsdf.alpha = 0.0f;
+        bsdf.alpha_weight = color.xyz();
+    }
+};
+
+// Translucent and velvet: the diffuse lobe (device/nodes/nodes.hpp:1829-1847, 1871-1893)
+struct BsdfTranslucent {
+    vec4 color;
+    vec3 normal;
+    float weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        bsdf.diffuse.diffuse_weight = color.xyz();
+        bsdf.diffuse.sheen_weight = vec3(0.0f);
+        bsdf.diffuse.roughness = 1.0f;
+        bsdf.diffuse.translucent = true;
+        bsdf.normal = normal;
+        bsdf.lobe_prob[0] = 1.0f;
+    }
+};
+struct BsdfVelvet {
+    vec4 color;
+    vec3 normal;
+    float sigma = 0.0f, weight = 0.0f;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        bsdf.diffuse.diffuse_weight = color.xyz();
+        bsdf.diffuse.sheen_weight = vec3(0.0f);
+        bsdf.diffuse.roughness = sigma;
+        bsdf.diffuse.translucent = false;
+        bsdf.normal = normal;
+        bsdf.lobe_prob[0] = 1.0f;
+    }
+};
+
 // Outputs (device/nodes/nodes.hpp:1976-2005)
 struct OutputMaterial {
     vec3 displacement;

@@ -256,6 +256,70 @@ class Codegen {
             inv.kernel = "AddShader";
         } else if (t == "TEX_SKY_ANALYTIC") {
             inv.kernel = "TexSkyAnalytic";
+        } else if (t == "LIGHT_PATH") {
+            inv.kernel = "LightPath";
+        } else if (t == "VECT_MATH") {
+            const std::string op = enum_token(param_str(node, "operation", "ADD"));
+            static const std::set<std::string> ops = {"ADD",     "SUBTRACT", "MULTIPLY", "MULTIPLY_ADD", "NORMALIZE",
+                                                      "SCALE",   "MINIMUM",  "LENGTH",   "DOT_PRODUCT"};
+            if (!ops.count(op)) throw CodegenError("unsupported vector math operation '" + op + "'");
+            inv.kernel = "VectorMath<kVectorMath::OPERATION_" + op + ">";
+        } else if (t == "TEX_CHECKER") {
+            inv.kernel = "TexChecker";
+        } else if (t == "SEPXYZ") {
+            inv.kernel = "SeparateVector";
+        } else if (t == "COMBXYZ") {
+            inv.kernel = "CombineVector";
+        } else if (t == "MAP_RANGE") {
+            if (enum_token(param_str(node, "data_type", "FLOAT")) != "FLOAT" ||
+                enum_token(param_str(node, "interpolation_type", "LINEAR")) != "LINEAR")
+                throw CodegenError("map range: only float data with linear interpolation is supported");
+            inv.kernel = std::string("MapRange<") + (param_bool(node, "clamp") ? "true" : "false") + ">";
+        } else if (t == "MAPPING") {
+            inv.kernel = "Mapping";
+        } else if (t == "SEPARATE_COLOR" || t == "COMBINE_COLOR") {
+            const std::string mode = enum_token(param_str(node, "mode", "RGB"));
+            if (mode != "RGB" && mode != "HSV") throw CodegenError("unsupported color mode '" + mode + "'");
+            inv.kernel = t == "SEPARATE_COLOR" ? "SeparateColor<kSeparateColor::MODE_" + mode + ">"
+                                                : "CombineColor<kCombineColor::MODE_" + mode + ">";
+        } else if (t == "HUE_SAT") {
+            inv.kernel = "HueSaturation";
+        } else if (t == "BRIGHTCONTRAST") {
+            inv.kernel = "BrightnessContrast";
+        } else if (t == "GAMMA") {
+            inv.kernel = "Gamma";
+        } else if (t == "INVERT") {
+            inv.kernel = "ColorInvert";
+        } else if (t == "RGBTOBW") {
+            inv.kernel = "RGBToBW";
+        } else if (t == "FRESNEL") {
+            inv.kernel = "Fresnel";
+        } else if (t == "LAYER_WEIGHT") {
+            inv.kernel = "LayerWeight";
+        } else if (t == "MIX") {
+            inv.kernel = std::string("Mix<") + (param_bool(node, "clamp_factor") ? "true" : "false") + ", " +
+                         (param_bool(node, "clamp_result") ? "true" : "false") + ", kMix::FACTOR_MODE_" +
+                         enum_token(param_str(node, "factor_mode", "UNIFORM")) + ">";
+        } else if (t == "TEX_GRADIENT") {
+            const std::string gt = enum_token(param_str(node, "type", "LINEAR"));
+            static const std::set<std::string> types = {"LINEAR", "SPHERICAL", "EASING", "QUADRATIC",
+                                                        "QUADRATIC_SPHERE"};
+            if (!types.count(gt)) throw CodegenError("unsupported gradient type '" + gt + "'");
+            inv.kernel = "TexGradient<kTexGradient::TYPE_" + gt + ">";
+        } else if (t == "BSDF_GLOSSY") {
+            inv.kernel = "BsdfGlossy";
+        } else if (t == "BSDF_ANISOTROPIC") {
+            inv.kernel = "BsdfAnisotropic";
+        } else if (t == "BSDF_GLASS") {
+            inv.kernel = "BsdfGlass";
+        } else if (t == "BSDF_REFRACTION") {
+            inv.kernel = "BsdfRefraction";
+        } else if (t == "BSDF_TRANSPARENT") {
+            inv.kernel = "BsdfTransparent";
+        } else if (t == "BSDF_TRANSLUCENT") {
+            inv.kernel = "BsdfTranslucent";
+        } else if (t == "BSDF_VELVET") {
+            inv.kernel = "BsdfVelvet";
         } else {
             throw CodegenError("unsupported node type '" + t + "'");
         }

@@ -1,0 +1,223 @@
+"""Converter nodes beyond the benchmark graphs (device/nodes/nodes.hpp), in
+closed form through the oracle: a camera looks at an emitting quad whose
+Emission colour is the node's output, in a black world, so every pixel of a
+one-bounce render is that output (colour x strength 1).  The expected values
+are computed here in float64 from the node's definition in Blender/Cycles
+terms (with the reference's simplifications noted per case).
+
+Also: the codegen accepts every newly mapped node type and rejects the
+variants the reference's enums do not have."""
+import colorsys
+import math
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from oracle import binding
+
+
+def _in(t, value=None, link=None):
+    return scenes._in(t, value, link)
+
+
+def emitter_scene(nodes, out_link):
+    """`nodes` plus an Emission fed from out_link; one quad facing the camera."""
+    graph = dict(nodes)
+    graph["Emission"] = {"type": "EMISSION", "parameters": {},
+                         "inputs": {"Color": _in("RGBA", [0, 0, 0, 1], link=out_link), "Strength": _in("VALUE", 1.0),
+                                    "Weight": _in("VALUE", 0.0)}}
+    graph["Material Output"] = scenes.output_material(("Emission", "Emission"))
+    d = scenes.SceneDescription("node")
+    d.surface_materials = [{"name": "node", "nodes": graph}]
+    d.light_materials = [scenes.background_world("world", (0.0, 0.0, 0.0), 1.0)]
+    d.world_shader = 0
+    d.add_shape(scenes.merge([scenes.quad((-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, 2, 0), 0)]))
+    d.add_entity(0)
+    d.camera = dict(transform=scenes.look_at((0.0, 0.0, 1.0), (0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)),
+                    near=1e-3, far=1e4, focal=2.0)
+    d.envmap_resolution = 16
+    return d
+
+
+def render_value(nodes, out_link):
+    orc = binding.OracleScene(emitter_scene(nodes, out_link))
+    orc.build_envmap()
+    img, _ = orc.render(4, 4, 1, frames=1)
+    orc.close()
+    return img[..., :3].astype(np.float64)
+
+
+def node(t, inputs, params=None):
+    return {"type": t, "parameters": params or {}, "inputs": inputs}
+
+
+def check(img, rgb, rel=2e-6):
+    want = np.broadcast_to(np.asarray(rgb, np.float64), img.shape)
+    assert np.allclose(img, want, rtol=rel, atol=1e-7), (img[0, 0], rgb)
+
+
+C = [0.8, 0.3, 0.1, 1.0]
+
+
+def test_invert():
+    img = render_value({"N": node("INVERT", {"Color": _in("RGBA", C), "Fac": _in("VALUE", 0.75)})}, ("N", "Color"))
+    check(img, [c * 0.25 + (1 - c) * 0.75 for c in C[:3]])
+
+
+def test_gamma():
+    img = render_value({"N": node("GAMMA", {"Color": _in("RGBA", C), "Gamma": _in("VALUE", 2.2)})}, ("N", "Color"))
+    check(img, [c ** 2.2 for c in C[:3]])
+    img = render_value({"N": node("GAMMA", {"Color": _in("RGBA", C), "Gamma": _in("VALUE", 0.0)})}, ("N", "Color"))
+    check(img, [1.0, 1.0, 1.0])
+
+
+def test_brightness_contrast():
+    img = render_value({"N": node("BRIGHTCONTRAST", {"Color": _in("RGBA", C), "Bright": _in("VALUE", 0.1),
+                                                     "Contrast": _in("VALUE", 0.4)})}, ("N", "Color"))
+    check(img, [max(1.4 * c + (0.1 - 0.2), 0.0) for c in C[:3]])
+
+
+def test_rgb_to_bw():
+    img = render_value({"N": node("RGBTOBW", {"Color": _in("RGBA", C)})}, ("N", "Val"))
+    check(img, [0.2126 * C[0] + 0.7152 * C[1] + 0.0722 * C[2]] * 3)
+
+
+def test_hue_saturation():
+    # Cycles' hue/saturation: hue shifted by (hue - 0.5), saturation scaled and
+    # clamped, value scaled, then blended with the input by fac
+    hue, sat, val, fac = 0.6, 0.7, 1.3, 0.8
+    img = render_value({"N": node("HUE_SAT", {"Color": _in("RGBA", C), "Hue": _in("VALUE", hue),
+                                              "Saturation": _in("VALUE", sat), "Value": _in("VALUE", val),
+                                              "Fac": _in("VALUE", fac)})}, ("N", "Color"))
+    h, s, v = colorsys.rgb_to_hsv(*C[:3])
+    r = colorsys.hsv_to_rgb((h + hue + 0.5) % 1.0, min(max(s * sat, 0.0), 1.0), v * val)
+    check(img, [(1 - fac) * a + fac * max(b, 0.0) for a, b in zip(C[:3], r)], rel=1e-5)
+
+
+def test_separate_and_combine():
+    nodes = {"S": node("SEPARATE_COLOR", {"Color": _in("RGBA", C)}, {"mode": "RGB"}),
+             "V": node("COMBXYZ", {"X": _in("VALUE", 0.0, link=("S", "Blue")), "Y": _in("VALUE", 0.0, link=("S", "Red")),
+                                   "Z": _in("VALUE", 0.0, link=("S", "Green"))}),
+             "W": node("SEPXYZ", {"Vector": _in("VECTOR", [0, 0, 0], link=("V", "Vector"))}),
+             "K": node("COMBINE_COLOR", {"Red": _in("VALUE", 0.0, link=("W", "Y")),
+                                         "Green": _in("VALUE", 0.0, link=("W", "X")),
+                                         "Blue": _in("VALUE", 0.0, link=("W", "Z"))}, {"mode": "RGB"})}
+    check(render_value(nodes, ("K", "Color")), [C[0], C[2], C[1]])
+    # separating HSV gives (h, s, v)
+    img = render_value({"S": node("SEPARATE_COLOR", {"Color": _in("RGBA", C)}, {"mode": "HSV"}),
+                        "K": node("COMBINE_COLOR", {"Red": _in("VALUE", 0.0, link=("S", "Red")),
+                                                    "Green": _in("VALUE", 0.0, link=("S", "Green")),
+                                                    "Blue": _in("VALUE", 0.0, link=("S", "Blue"))},
+                                  {"mode": "RGB"})}, ("K", "Color"))
+    check(img, list(colorsys.rgb_to_hsv(*C[:3])), rel=1e-5)
+
+
+@pytest.mark.parametrize("clamp", [False, True])
+def test_map_range(clamp):
+    img = render_value({"N": node("MAP_RANGE", {"Value": _in("VALUE", 0.9), "From Min": _in("VALUE", 0.2),
+                                                "From Max": _in("VALUE", 0.6), "To Min": _in("VALUE", 0.1),
+                                                "To Max": _in("VALUE", 0.3)},
+                                  {"clamp": clamp, "data_type": "FLOAT", "interpolation_type": "LINEAR"})},
+                       ("N", "Result"))
+    t = (0.9 - 0.2) / (0.6 - 0.2)
+    if clamp:
+        t = min(t, 1.0)
+    check(img, [0.1 + (0.3 - 0.1) * t] * 3, rel=1e-5)
+
+
+@pytest.mark.parametrize("op,want", [
+    ("ADD", [0.5, 0.9, 0.4]), ("SUBTRACT", [-0.1, 0.1, 0.0]), ("MULTIPLY", [0.06, 0.2, 0.04]),
+    ("MULTIPLY_ADD", [0.06 + 0.05, 0.2 + 0.05, 0.04 + 0.05]), ("SCALE", [0.6, 1.5, 0.6]),
+    ("MINIMUM", [0.2, 0.4, 0.2]), ("NORMALIZE", list(np.array([0.2, 0.5, 0.2]) / np.linalg.norm([0.2, 0.5, 0.2]))),
+    ("DOT_PRODUCT", [0.06 + 0.2 + 0.04] * 3)])
+def test_vector_math(op, want):
+    nodes = {"N": node("VECT_MATH", {"Vector": _in("VECTOR", [0.2, 0.5, 0.2]), "Vector_001": _in("VECTOR", [0.3, 0.4, 0.2]),
+                                     "Vector_002": _in("VECTOR", [0.05, 0.05, 0.05]), "Scale": _in("VALUE", 3.0)},
+                       {"operation": op})}
+    check(render_value(nodes, ("N", "Vector")), want, rel=1e-5)
+
+
+def test_vector_length():
+    nodes = {"N": node("VECT_MATH", {"Vector": _in("VECTOR", [0.2, 0.5, 0.2])}, {"operation": "LENGTH"})}
+    check(render_value(nodes, ("N", "Value")), [math.sqrt(0.04 + 0.25 + 0.04)] * 3, rel=1e-6)
+
+
+def test_mapping_rotates_scales_and_moves():
+    rot = [0.3, -0.5, 1.1]
+    nodes = {"N": node("MAPPING", {"Vector": _in("VECTOR", [0.3, 0.2, 0.1]), "Location": _in("VECTOR", [0.1, 0.2, 0.3]),
+                                   "Rotation": _in("VECTOR", rot), "Scale": _in("VECTOR", [2.0, 1.0, 0.5])},
+                       {"type": "MAPPING"})}
+    img = render_value(nodes, ("N", "Vector"))
+    cx, cy, cz = (math.cos(a) for a in rot)
+    sx, sy, sz = (math.sin(a) for a in rot)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    want = Rz @ Ry @ Rx @ (np.array([0.3, 0.2, 0.1]) * [2.0, 1.0, 0.5]) + [0.1, 0.2, 0.3]
+    check(img, np.where(np.abs(want) < 1e-7, 0.0, want), rel=1e-5)
+
+
+@pytest.mark.parametrize("mode", ["UNIFORM", "NON_UNIFORM"])
+def test_mix(mode):
+    nodes = {"N": node("MIX", {"Factor_Float": _in("VALUE", 0.25), "Factor_Vector": _in("VECTOR", [0.1, 0.5, 0.9]),
+                               "A_Vector": _in("VECTOR", [0.2, 0.4, 0.6]), "B_Vector": _in("VECTOR", [1.0, 0.0, 0.5]),
+                               "A_Color": _in("RGBA", C), "B_Color": _in("RGBA", [0.0, 1.0, 0.5, 1.0])},
+                       {"type": "VECTOR", "clamp_factor": True, "clamp_result": False, "factor_mode": mode})}
+    img = render_value(nodes, ("N", "Result_Vector"))
+    a, b = np.array([0.2, 0.4, 0.6]), np.array([1.0, 0.0, 0.5])
+    f = np.array([0.25] * 3) if mode == "UNIFORM" else np.array([0.1, 0.5, 0.9])
+    check(img, a + (b - a) * f, rel=1e-5)
+    check(render_value(nodes, ("N", "Result_Color")), [0.75 * c + 0.25 * d for c, d in zip(C[:3], [0.0, 1.0, 0.5])],
+          rel=1e-5)
+
+
+@pytest.mark.parametrize("gtype", ["LINEAR", "SPHERICAL"])
+def test_gradient(gtype):
+    v = [0.3, 0.4, 0.0]
+    nodes = {"N": node("TEX_GRADIENT", {"Vector": _in("VECTOR", v)}, {"type": gtype})}
+    want = 0.3 if gtype == "LINEAR" else 0.5
+    check(render_value(nodes, ("N", "Fac")), [want] * 3, rel=1e-6)
+
+
+def test_checker_alternates_on_the_generated_grid():
+    nodes = {"N": node("TEX_CHECKER", {"Vector": _in("VECTOR", [0, 0, 0]), "Scale": _in("VALUE", 2.0),
+                                       "Color1": _in("RGBA", [1.0, 0.0, 0.0, 1.0]),
+                                       "Color2": _in("RGBA", [0.0, 0.0, 1.0, 1.0])})}
+    orc = binding.OracleScene(emitter_scene(nodes, ("N", "Color")))
+    orc.build_envmap()
+    img, _ = orc.render(32, 32, 1, frames=1)
+    orc.close()
+    red, blue = img[..., 0] > 0.5, img[..., 2] > 0.5
+    assert np.all(red ^ blue) and red.any() and blue.any()
+
+
+@pytest.mark.parametrize("t,params", [("VECT_MATH", {"operation": "CROSS_PRODUCT"}),
+                                      ("MAP_RANGE", {"data_type": "FLOAT", "interpolation_type": "STEPPED"}),
+                                      ("SEPARATE_COLOR", {"mode": "HSL"}), ("TEX_GRADIENT", {"type": "RADIAL"})])
+def test_codegen_rejects_what_the_reference_lacks(t, params):
+    d = emitter_scene({"N": node(t, {}, params)}, ("N", "Color"))
+    with pytest.raises(api.RmError):
+        api.codegen(d.materials_json())
+
+
+def test_shader_nodes_compile():
+    """One material per BSDF node plus a light-path switch; the oracle's g++
+    build of the generated code compiles every node."""
+    d = emitter_scene({"LP": node("LIGHT_PATH", {})}, ("LP", "Is Camera Ray"))
+    kinds = ["BSDF_GLOSSY", "BSDF_ANISOTROPIC", "BSDF_GLASS", "BSDF_REFRACTION", "BSDF_TRANSPARENT",
+             "BSDF_TRANSLUCENT", "BSDF_VELVET"]
+    for t in kinds:
+        d.surface_materials.append({"name": t, "nodes": {
+            "B": node(t, {"Color": _in("RGBA", C)}),
+            "Material Output": scenes.output_material(("B", "BSDF"))}})
+    src = api.codegen(d.materials_json())
+    for k in ["LightPath", "BsdfGlossy", "BsdfAnisotropic", "BsdfGlass", "BsdfRefraction", "BsdfTransparent",
+              "BsdfTranslucent", "BsdfVelvet"]:
+        assert k in src
+    orc = binding.OracleScene(d)
+    orc.build_envmap()
+    img, _ = orc.render(4, 4, 1, frames=1)
+    orc.close()
+    check(img[..., :3].astype(np.float64), [1.0, 1.0, 1.0])  # seen by a camera ray
