@@ -71,6 +71,16 @@ typedef struct rmh_ply_mesh {
 int rmh_ply_read(const char* path, rmh_ply_mesh* out, char** out_error);
 void rmh_ply_free(rmh_ply_mesh* mesh);
 
+/* Writes an RGBA float image (width*height*4, row 0 first) as an OpenEXR
+ * file: RGB times norm, alpha 1, as 32-bit float channels A, B, G, R,
+ * uncompressed scanlines.  Replaces Renderer.saveFrame's EXR output
+ * (host/Renderer.swift:804-807, utils/Metal.swift:75-124), which the caller
+ * reproduces by passing the accumulation buffer and norm = 1 / (frames - 1)
+ * when accumulating, else 1.  Returns 0, or -1 and a malloc'd message (free
+ * with rmh_free). */
+int rmh_save_exr(const char* path, const float* rgba, uint32_t width, uint32_t height, float norm,
+                 char** out_error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,6 +194,7 @@ EXPORTS = {
     "rmh_free": (None, [C.c_void_p]),
     "rmh_ply_read": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "rmh_ply_free": (None, [C.c_void_p]),
+    "rmh_save_exr": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_void_p)]),
     "rmh_jit_precompile": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "rmh_bvh8_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.c_char_p, C.c_size_t]),
@@ -269,6 +270,20 @@ def read_ply(path):
                     bounds_max=np.array(mesh.bounds_max[:], np.float32))
     finally:
         LIB.rmh_ply_free(C.byref(mesh))
+
+
+def save_exr(path, rgba, norm=1.0):
+    """Writes an (h, w, 4) float image as OpenEXR (rmh_save_exr): RGB * norm,
+    alpha 1."""
+    img = np.ascontiguousarray(rgba, np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError("expected an (h, w, 4) image")
+    err = C.c_void_p()
+    if LIB.rmh_save_exr(os.fsencode(path), img.ctypes.data, img.shape[1], img.shape[0], float(norm),
+                        C.byref(err)) != 0:
+        msg = C.cast(err, C.c_char_p).value.decode() if err.value else "EXR write failed"
+        LIB.rmh_free(err)
+        raise RmError(msg)
 
 
 def identifier(name):
@@ -440,6 +455,17 @@ class Renderer:
     @property
     def frame_index(self):
         return LIB.rm_renderer_frame_index(self.handle)
+
+    def save_frame(self, path):
+        """Renderer.saveFrame (Renderer.swift:804-807): the accumulation buffer
+        as OpenEXR, scaled by 1 / (frame index - 1) when accumulating, as the
+        reference scales it (one frame fewer than normalizeImage divides by)."""
+        norm = 1.0
+        if self.uniforms.accumulate:
+            if self.frame_index < 2:
+                raise RmError("save_frame needs at least two accumulated frames (the scale is 1 / (frames - 1))")
+            norm = float(np.float32(1.0) / np.float32(self.frame_index - 1))
+        save_exr(path, self.image(), norm)
 
     def set_cache(self, cache, mode=1):
         """Attach a RadianceCache (mode 1: query and train, 2: query only) or detach (None)."""

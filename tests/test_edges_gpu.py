@@ -97,3 +97,22 @@ def test_invalid_renderer_arguments_are_rejected(box, W, H, depth, spl):
     desc, scene, orc = box
     with pytest.raises(RuntimeError):
         api.Renderer(scene, W, H, max_depth=depth, samples_per_launch=spl)
+
+
+def test_save_frame_writes_the_scaled_accumulation(box, tmp_path):
+    """Renderer.saveFrame: the accumulation buffer as OpenEXR, RGB scaled by
+    1 / (frames - 1) as the reference scales it, alpha 1."""
+    from .test_exr_cpu import read_exr
+    desc, scene, orc = box
+    r = api.Renderer(scene, 16, 12, max_depth=4)
+    with pytest.raises(api.RmError):
+        r.save_frame(str(tmp_path / "early.exr"))
+    r.execute(3)
+    path = str(tmp_path / "frame.exr")
+    r.save_frame(path)
+    img = r.image()
+    _, ch = read_exr(path)
+    norm = np.float32(1.0) / np.float32(2.0)
+    for k, n in enumerate("RGB"):
+        assert np.array_equal(ch[n], (img[..., k] * norm).astype(np.float32))
+    assert np.all(ch["A"] == 1.0) and img[..., :3].max() > 0.0
