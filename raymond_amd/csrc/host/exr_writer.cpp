@@ -56,7 +56,9 @@ struct Bytes {
 
 void write_exr(const char* path, const float* rgba, uint32_t width, uint32_t height, float norm) {
     if (width == 0 || height == 0) throw std::runtime_error("empty image");
-    if (width > 0x7fffffffu || height > 0x7fffffffu) throw std::runtime_error("image too large");
+    // a block's byte count (16 bytes per pixel of a scanline) and the window
+    // corners are 32-bit signed integers in the file
+    if (16ull * width > 0x7fffffffull || height > 0x7fffffffu) throw std::runtime_error("image too large");
     static const char* kChannels[4] = {"A", "B", "G", "R"};  // sorted by name
     static const int kSource[4] = {3, 2, 1, 0};              // rgba component of each channel
 
