@@ -387,11 +387,15 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
     }
 }
 
-// At least 4 waves per SIMD: large inlined graph sets would otherwise take
-// ~250 VGPRs (one or two waves); the spills this forces cost less than the
-// latency it hides (32 random 20-60 node graphs at 1080p: 1450 -> 1583
-// Mrays/s; 3 waves: 1559).
-constexpr int kShadeMinWaves = 4;
+// Waves per SIMD the shading kernels are held to.  Unbounded, large inlined
+// graph sets take ~250 VGPRs (one or two waves); the spills a bound forces
+// cost less than the latency it hides.  The plain kernel runs best at 5
+// (32 random 20-60 node graphs at 1080p: 1803 -> 1826 Mrays/s against 4;
+// sphere field 2772 -> 2798; Cornell box 4719 -> 4827), the batch kernel,
+// which carries per-job state, at 4 (3605 against 3355 Mrays/s at 5); the
+// cache kernel keeps 4 with it.
+constexpr int kShadeMinWaves = 5;
+constexpr int kShadeWideMinWaves = 4;
 
 extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_kernel(ShadeArgs a) {
     shade_queue<false, false>(a);
@@ -400,7 +404,7 @@ extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_sha
 #if defined(RM_CACHE_KERNEL)
 // Renderers with a radiance cache: lookups, training paths and their
 // radiance sums.  Compiled only into the modules of scenes a cache renders.
-extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_cache_kernel(ShadeArgs a) {
+extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeWideMinWaves) void rm_shade_cache_kernel(ShadeArgs a) {
     shade_queue<false, true>(a);
 }
 #endif
@@ -408,7 +412,7 @@ extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_sha
 #if defined(RM_BATCH_KERNEL)
 // Batched jobs (a.batch.jobs non-null): per-job lights, depth and path counts.
 // Compiled only into the modules of batches.
-extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeMinWaves) void rm_shade_batch_kernel(ShadeArgs a) {
+extern "C" __global__ __launch_bounds__(kShadeBlock, kShadeWideMinWaves) void rm_shade_batch_kernel(ShadeArgs a) {
     shade_queue<true, false>(a);
 }
 #endif
