@@ -209,10 +209,12 @@ struct RayState {
         ihits = (oct & 4) ? ((ihits & 0x0fu) << 4) | (ihits >> 4) : ihits;
         const int near_k = (int)(near_key & 7u) ^ oct;
         cur = make_uint2(q1.x, (ihits << 24) | imask);
-        // descend into the nearest child first (closest hit: best_t
-        // shrinks sooner; any hit: near occluders are found sooner); the
-        // other hit children wait on the stack as one group
-        if (ihits & (ihits - 1u)) {
+        // closest hit: descend into the nearest child first (best_t shrinks
+        // sooner), the other hit children wait on the stack as one group.
+        // Any hit keeps the octant order: choosing the nearest child costs
+        // more there than it saves (shadow rays on the sphere field 0.43 ->
+        // 0.40 ms per spp without it).
+        if (!ANY && (ihits & (ihits - 1u))) {
             stk.store(sp, make_uint2(q1.x, ((ihits & ~(1u << near_k)) << 24) | imask));
             sp++;
             cur.y = (1u << (24 + near_k)) | imask;
