@@ -391,7 +391,8 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
 // graph sets take ~250 VGPRs (one or two waves); the spills a bound forces
 // cost less than the latency it hides.  The plain kernel runs best at 5
 // (32 random 20-60 node graphs at 1080p: 1803 -> 1826 Mrays/s against 4;
-// sphere field 2772 -> 2798; Cornell box 4719 -> 4827), the batch kernel,
+// sphere field 2772 -> 2798; Cornell box 4719 -> 4827; 6 waves lose on all
+// three, e.g. 2831 -> 2783 on the sphere field), the batch kernel,
 // which carries per-job state, at 4 (3605 against 3355 Mrays/s at 5); the
 // cache kernel keeps 4 with it.
 constexpr int kShadeMinWaves = 5;
