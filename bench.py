@@ -7,6 +7,12 @@ depth 12.  One step = one sample per pixel (2,073,600 paths) traced through
 all bounces.  Mrays/s counts closest-hit plus shadow rays traced, the
 reference's own accounting (host/Renderer.swift:190-196,249-253).
 
+The same line carries `cache_vs_plain`, the metric's second half: the Cornell box
+at 1920x1080 rendered by the plain tracer and by the tracer with the learned
+radiance cache, RMSE against a 16384-spp FP64 reference at power-of-two spp,
+and the cached tracer's time to the plain tracer's 64-spp RMSE over the plain
+time (`time_ratio`).
+
 `--workload graphs` runs the shader-graph workload instead: the Cornell-box geometry
 with 32 random node-graph materials, depth 8, shading sorted by material.
 
@@ -121,19 +127,23 @@ def queue_frames(args, W, H):
     return args.spp_per_launch if args.spp_per_launch > 0 else max(1, (1 << 24) // (W * H))
 
 
-def bench_cache(args):
-    """Radiance cache against the plain tracer: the Cornell box at full HD, depth 12, roulette from
-    bounce 3.  A converged FP64 render of the plain tracer is the reference;
-    the plain tracer and the cached tracer (online training on 1 path in 16,
-    cold start) each render `steps` spp, and the RMSE against the reference is
-    recorded at power-of-two sample counts with the GPU time spent so far."""
+def cache_vs_plain(args, dev=None):
+    """Radiance cache against the plain tracer: the Cornell box at full HD,
+    depth 12, roulette from bounce 3.  A converged FP64 render of the plain
+    tracer is the reference; the plain tracer and the cached tracer (online
+    training on 1 path in 16, cold start) each render `cache_spp` spp, and the
+    RMSE against the reference is recorded at power-of-two sample counts with
+    the time spent so far.  Returns the comparison as a dict, with the
+    cached tracer's time to the plain tracer's final RMSE over the plain time
+    (`time_ratio`)."""
     import numpy as np
     import raymond_amd.api as api
     import raymond_amd.scenes as scenes
 
-    W, H = args.width, args.height
-    D = args.depth or 12
-    dev = api.Device(0)
+    W, H = (args.width, args.height) if args.workload == "cache" else (1920, 1080)
+    D = (args.depth or 12) if args.workload == "cache" else 12
+    spp = args.steps if args.workload == "cache" else args.cache_spp
+    dev = dev or api.Device(0)
     desc = scenes.cornell_box(W, H)
     desc.envmap_resolution = 64  # black world
     scene = scenes.build_device_scene(dev, desc)
@@ -154,6 +164,7 @@ def bench_cache(args):
         r = api.Renderer(scene, W, H, max_depth=D, samples_per_launch=spl)
         r.set_uniforms(rr_depth=3)
         cache = None
+
         def new_cache():
             return api.RadianceCache(dev, seed=1, batch_size=args.cache_batch, steps_per_launch=args.cache_steps)
 
@@ -168,8 +179,8 @@ def bench_cache(args):
             r.set_cache(cache, mode=1)
         points, done, spent, rays = [], 0, 0.0, 0
         k = 1
-        while done < args.steps:
-            n = min(k, args.steps) - done
+        while done < spp:
+            n = min(k, spp) - done
             t = time.perf_counter()
             s = r.execute(n)
             spent += time.perf_counter() - t
@@ -208,26 +219,65 @@ def bench_cache(args):
     target = plain["points"][-1]["rmse"]
     t_plain = plain["points"][-1]["seconds"]
     t_cache = time_to(cached["points"], target)
+    return {
+        "workload": "radiance cache: Cornell box %dx%d, depth %d, roulette from bounce 3, hash grid 16x2 FP16 x "
+                    "2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 trains, %d training steps of %d "
+                    "samples per spp, %d spp each; reference: plain tracer, %d spp, FP64 accumulation"
+                    % (W, H, D, args.cache_steps, args.cache_batch, spp, args.ref_spp),
+        "spp": spp,
+        "plain": plain,
+        "cached": cached,
+        "plain_rmse": target,
+        "cached_rmse_at_equal_spp": cached["points"][-1]["rmse"],
+        "plain_seconds": t_plain,
+        "cache_seconds_to_plain_rmse": None if t_cache is None else round(t_cache, 4),
+        "time_ratio": None if t_cache is None else round(t_cache / t_plain, 4),
+        "target_time_ratio": 0.5,
+    }
+
+
+def bench_cache(args):
+    """The cache comparison as the headline line (`--workload cache`)."""
+    c = cache_vs_plain(args)
     out = {
         "metric": "Mrays/s + RMSE vs high-spp reference at equal time, cache vs plain",
-        "value": cached["mrays_per_s"],
+        "value": c["cached"]["mrays_per_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "radiance cache: Cornell box %dx%d, depth %d, roulette from bounce 3, "
-                               "hash grid 16x2 FP16 x 2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 "
-                               "trains, %d training steps of %d samples per spp; reference: plain tracer, %d spp, "
-                               "FP64 accumulation" % (W, H, D, args.cache_steps, args.cache_batch, args.ref_spp)},
-        "plain": plain,
-        "cached": cached,
-        "plain_rmse_at_steps": target,
-        "cache_seconds_to_plain_rmse": None if t_cache is None else round(t_cache, 4),
-        "time_ratio_cache_vs_plain": None if t_cache is None else round(t_cache / t_plain, 4),
+        "config": {"workload": c["workload"]},
+        "plain": c["plain"],
+        "cached": c["cached"],
+        "plain_rmse_at_steps": c["plain_rmse"],
+        "cache_seconds_to_plain_rmse": c["cache_seconds_to_plain_rmse"],
+        "time_ratio_cache_vs_plain": c["time_ratio"],
     }
     print(json.dumps(out), flush=True)
+
+
+def host_cpu():
+    """CPU model and core counts of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    # the host cores this job may use: the box allots a share of its CPUs to
+    # each GPU (OMP_NUM_THREADS names it), which bounds any worker pool here
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or allowed
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": allowed,
+            "job_share": min(share, allowed)}
 
 
 def main():
@@ -252,6 +302,10 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
     ap.add_argument("--cache-batch", type=int, default=8192, help="cache workload: samples per training step")
     ap.add_argument("--cache-steps", type=int, default=2, help="cache workload: training steps per sample")
+    ap.add_argument("--cache-spp", type=int, default=64,
+                    help="spp of each side of the cache-vs-plain comparison in the default line")
+    ap.add_argument("--no-cache-compare", action="store_true",
+                    help="leave the cache-vs-plain comparison out of the default line")
     ap.add_argument("--material-sort", type=int, default=1,
                     help="group hits by material (misses in a bin of their own) before shading")
     ap.add_argument("--inline-materials", type=int, default=-1,
@@ -369,7 +423,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import binding
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu_host = host_cpu()
+        threads = args.cpu_threads or cpu_host["job_share"]
         orc = binding.OracleScene(desc)
         pdfs, mip = scene.read_envmap()
         orc.set_envmap(pdfs, mip)
@@ -384,7 +439,8 @@ def main():
             sample = "%d full %dx%d frames (%d bench steps), 1 spp each, depth %d" % (frames, W, H, frames, D)
         _, res = orc.render(W, H, D, frames=frames, frame_begin=10_000, pixels=pix, threads=threads, rr_depth=rr_depth)
         cpu = {"value": res.total_rays / res.seconds / 1e6, "unit": "Mrays/s", "cores": int(res.threads_used),
-               "kind": "port", "sample": sample, "seconds": round(res.seconds, 2)}
+               "kind": "port", "sample": sample, "seconds": round(res.seconds, 2), "cpu_model": cpu_host["model"],
+               "host_logical_cpus": cpu_host["logical_cpus"], "host_affinity_cpus": cpu_host["affinity_cpus"]}
         print("cpu oracle: %.3f Mrays/s on %d threads (%.1fs)" % (cpu["value"], cpu["cores"], res.seconds),
               file=sys.stderr, flush=True)
 
@@ -425,6 +481,11 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
+    if loaded is None and not graphs and not cornell and not args.no_cache_compare:
+        # the metric's second half on the same device: the cache against the
+        # plain tracer at equal time (the sphere field's buffers are freed first)
+        del r, scene
+        out["cache_vs_plain"] = cache_vs_plain(args, dev)
     print(json.dumps(out), flush=True)
 
 

@@ -57,6 +57,7 @@ def lib():
                                        C.c_void_p, C.c_int, C.c_int]
         L.or_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_int]
         L.or_render.argtypes = [C.c_void_p, C.POINTER(RenderArgs), C.c_void_p, C.POINTER(RenderResult)]
+        L.or_render_f64.argtypes = [C.c_void_p, C.POINTER(RenderArgs), C.c_void_p, C.POINTER(RenderResult)]
         _LIB = L
     return _LIB
 
@@ -153,6 +154,19 @@ class OracleScene:
             args.num_pixels = pixels.size
         res = RenderResult()
         st = lib().or_render(self.handle, C.byref(args), image.ctypes.data, C.byref(res))
+        if st != 0:
+            raise RuntimeError("oracle render failed (%d)" % st)
+        return image, res
+
+    def render_f64(self, width, height, max_depth, frames=1, frame_begin=0, sampling_mode=2, rr_depth=0,
+                   threads=0):
+        """The same paths as render(), each sample summed in FP32 and the
+        samples accumulated in FP64 (the GPU's accumulate_f64 image)."""
+        image = np.zeros((height, width, 4), np.float64)
+        args = RenderArgs(width, height, max_depth, sampling_mode, rr_depth, frame_begin, frames, None, 0,
+                          threads, 0)
+        res = RenderResult()
+        st = lib().or_render_f64(self.handle, C.byref(args), image.ctypes.data, C.byref(res))
         if st != 0:
             raise RuntimeError("oracle render failed (%d)" % st)
         return image, res

@@ -351,8 +351,8 @@ struct ShadowRay {  // bridge/Ray.hpp:31-39
     vec3 weight;
 };
 
-void add(float* img, uint32_t W, uint32_t x, uint32_t y, vec3 c) {
-    float* p = img + 4 * ((size_t)y * W + x);
+// intersection.metal:84-87, shadow.metal:24-27: RGB += c, alpha += 1
+void add(float* p, vec3 c) {
     p[0] += c.x;
     p[1] += c.y;
     p[2] += c.z;
@@ -372,7 +372,8 @@ struct PathCounts {
 };
 
 // generateRays + depth loop of handleIntersections / handleShadowRays for one path
-void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_t y, uint32_t frame, float* img,
+// px: the RGBA sums of pixel (x, y)
+void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_t y, uint32_t frame, float* px,
                  PathCounts& cnt) {
     using namespace rm;
     const SceneView& ctx = S.view;
@@ -418,7 +419,7 @@ void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_
                                       ? 1.0f
                                       : mis_weight(ray.bsdf_pdf, lights_envmap_pdf(ctx, ray.direction));
                 evaluate_environment(ctx, shading);
-                add(img, W, ray.x, ray.y, mis * ray.weight * shading.material.emission);
+                add(px, mis * ray.weight * shading.material.emission);
             }
         } else {
             const PerInstance& inst = ctx.instances[isect.inst];
@@ -430,7 +431,7 @@ void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_
                 const float mis = (A.sampling_mode == 0u) || isinf_(ray.bsdf_pdf)
                                       ? 1.0f
                                       : mis_weight(ray.bsdf_pdf, lights_shape_pdf(ctx, inst, shading));
-                add(img, W, ray.x, ray.y, mis * ray.weight * shading.material.emission);
+                add(px, mis * ray.weight * shading.material.emission);
             }
             vec3 sh_normal;
             if (all_zero(shading.material.normal)) {
@@ -458,7 +459,7 @@ void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_
                         sray.weight = nw;
                     }
                 } else {
-                    add(img, W, ray.x, ray.y, ray.weight * contribution);
+                    add(px, ray.weight * contribution);
                 }
             }
             const BsdfSample smp =
@@ -498,7 +499,7 @@ void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_
             // handleShadowRays (shadow.metal:6-29): unoccluded -> add weight
             const Hit h = trace(S, sray.origin, sray.direction, sray.min_distance, sray.max_distance, true,
                                 A.brute_force != 0);
-            if (h.t < 0.0f) add(img, W, ray.x, ray.y, sray.weight);
+            if (h.t < 0.0f) add(px, sray.weight);
         }
         if (!emit_next) break;
         ray = next;
@@ -792,8 +793,8 @@ int or_trace_any(or_scene* s, const float* rays, uint32_t n, uint32_t* occluded,
     return 0;
 }
 
-int or_render(or_scene* s, const or_render_args* A, float* image, or_render_result* res) {
-    if (!s || !A || !image) return -1;
+static int render_impl(or_scene* s, const or_render_args* A, float* image, double* image64, or_render_result* res) {
+    if (!s || !A || (!image && !image64)) return -1;
     if (s->env_pdfs.empty()) return -2;
     const uint32_t np = A->pixels ? A->num_pixels : A->width * A->height;
     int nt = A->threads > 0 ? A->threads : (int)std::thread::hardware_concurrency();
@@ -816,7 +817,18 @@ int or_render(or_scene* s, const or_render_args* A, float* image, or_render_resu
             y = (tile / tiles_x) * 8 + within / 8;
             if (x >= A->width || y >= A->height) return;
         }
-        for (uint32_t f = 0; f < A->frames; f++) render_path(*s, *A, x, y, A->frame_begin + f, image, counts[th]);
+        const size_t at = 4 * ((size_t)y * A->width + x);
+        for (uint32_t f = 0; f < A->frames; f++) {
+            if (image) {
+                render_path(*s, *A, x, y, A->frame_begin + f, image + at, counts[th]);
+            } else {
+                // each sample summed in FP32 (its contributions in path order),
+                // the samples in FP64
+                float px[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                render_path(*s, *A, x, y, A->frame_begin + f, px, counts[th]);
+                for (int c = 0; c < 4; c++) image64[at + c] += (double)px[c];
+            }
+        }
     });
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (res) {
@@ -832,6 +844,16 @@ int or_render(or_scene* s, const or_render_args* A, float* image, or_render_resu
         res->threads_used = nt;
     }
     return 0;
+}
+
+int or_render(or_scene* s, const or_render_args* A, float* image, or_render_result* res) {
+    if (!image) return -1;
+    return render_impl(s, A, image, nullptr, res);
+}
+
+int or_render_f64(or_scene* s, const or_render_args* A, double* image, or_render_result* res) {
+    if (!image) return -1;
+    return render_impl(s, A, nullptr, image, res);
 }
 
 }  // extern "C"

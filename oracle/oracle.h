@@ -77,6 +77,9 @@ typedef struct or_render_result {
 
 /* Accumulates into image (width*height*4 floats, RGBA sums as the GPU). */
 int or_render(or_scene* s, const or_render_args* args, float* image, or_render_result* result);
+/* The same paths, each sample's contributions summed in FP32 and the samples
+ * accumulated in FP64 (width*height*4 doubles), as the GPU's FP64 image. */
+int or_render_f64(or_scene* s, const or_render_args* args, double* image, or_render_result* result);
 
 #ifdef __cplusplus
 }

@@ -41,29 +41,48 @@ def _run(cmd):
     return out
 
 
-def _compile(src):
+# Test build of the library with one LDS row of traversal stack, so that the
+# spill rows carry every deeper stack entry (tests/test_full_size_gpu.py).
+# The sources that include kernels/trace_api.h are compiled again with the
+# define; the other objects are shared with the main build.
+SPILL_LIB = os.path.join(HERE, "libraymond_amd_spill.so")
+SPILL_DEFINES = ["-DRM_LDS_STACK=1"]
+SPILL_SOURCES = ["kernels/trace.hip", "host/scene.cpp", "host/renderer.cpp", "host/batch.cpp",
+                 "host/capi.cpp"]
+
+
+def _compile(src, defines=(), tag=""):
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
+    obj = os.path.join(BUILD, tag + src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + list(defines) + ["-c", path, "-o", obj]
     else:
-        cmd = ["g++"] + HOST_FLAGS + ["-c", path, "-o", obj]
+        cmd = ["g++"] + HOST_FLAGS + list(defines) + ["-c", path, "-o", obj]
     _run(cmd)
     return obj
+
+
+def _link(objs, out):
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, out)
 
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     _run([sys.executable, os.path.join(CSRC, "embed_headers.py"), os.path.join(CSRC, "device"),
           os.path.join(CSRC, "host", "embedded_headers.inc")])
+    sources = KERNELS + HOST
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, KERNELS + HOST))
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
-    os.replace(tmp, LIB)
+        objs = list(ex.map(_compile, sources))
+        spill_objs = list(ex.map(lambda src: _compile(src, SPILL_DEFINES, "spill_"), SPILL_SOURCES))
+    _link(objs, LIB)
+    by_src = dict(zip(sources, objs))
+    by_src.update(zip(SPILL_SOURCES, spill_objs))
+    _link([by_src[src] for src in sources], SPILL_LIB)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", SPILL_LIB)
     return LIB
 
 
@@ -72,8 +91,8 @@ JIT_CACHE = os.path.join(HERE, "jitcache")
 
 def warm_jit_cache(verbose=False, big=False):
     """Compiles the material modules of the scenes the tests, smoke check and
-    default benchmark load into jitcache/ beside the library, which the JIT
-    reads before compiling; `big` adds the 32-graph scene in both forms.
+    benchmark workloads load into jitcache/ beside the library, which the JIT
+    reads before compiling; `big` adds the 32-graph scene's out-of-line form.
     hiprtc serialises compilations inside one process, so each module gets a
     process of its own.  Returns {scene: seconds}."""
     import json

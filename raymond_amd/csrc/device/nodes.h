@@ -146,6 +146,12 @@ struct TexNoise {
                      fractal_noise(p + off2(3.0f), detail, roughness), 1.0f);
         fac = color.x;
     }
+    RM_HD static vec4 off4(float seed) {
+        return vec4(100.0f + hash_float2_to_float(vec2(seed, 0.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 1.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 2.0f)) * 100.0f,
+                    100.0f + hash_float2_to_float(vec2(seed, 3.0f)) * 100.0f);
+    }
     RM_HD void tex3(vec3 p) {
         if (distortion != 0.0f)
             p = p + distortion * vec3(snoise3(p + off3(0.0f)), snoise3(p + off3(1.0f)), snoise3(p + off3(2.0f)));
@@ -154,13 +160,23 @@ struct TexNoise {
         fac = color.x;
     }
 
-    // (device/nodes/nodes.hpp:881-1011); the 4-D variant falls back to 3-D
+    RM_HD void tex4(vec4 p) {
+        if (distortion != 0.0f)
+            p = p + distortion * vec4(snoise4(p + off4(0.0f)), snoise4(p + off4(1.0f)), snoise4(p + off4(2.0f)),
+                                      snoise4(p + off4(3.0f)));
+        color = vec4(fractal_noise(p, detail, roughness), fractal_noise(p + off4(4.0f), detail, roughness),
+                     fractal_noise(p + off4(5.0f), detail, roughness), 1.0f);
+        fac = color.x;
+    }
+
+    // (device/nodes/nodes.hpp:881-1011)
     RM_HD void compute(const SceneView&, ShadingContext&) {
         const vec3 p = vector * scale;
         const float ws = w * scale;
         switch (Dimension) {
         case kTexNoise::DIMENSION_1D: tex1(ws); break;
         case kTexNoise::DIMENSION_2D: tex2(p.xy()); break;
+        case kTexNoise::DIMENSION_4D: tex4(vec4(p.x, p.y, p.z, ws)); break;
         default: tex3(p); break;
         }
     }

@@ -142,6 +142,28 @@ RM_HD float perlin_3d(float x, float y, float z) {
                    grad3(H(X + 1, Y + 1, Z + 1), fx - 1.0f, fy - 1.0f, fz - 1.0f), u, v, w);
 }
 
+// four-linear blend of two tri_mix cubes along w (device/utils/noise.hpp:218-241)
+RM_HD float quad_mix(float v0, float v1, float v2, float v3, float v4, float v5, float v6, float v7, float v8,
+                     float v9, float v10, float v11, float v12, float v13, float v14, float v15, float x, float y,
+                     float z, float w) {
+    return mixf(tri_mix(v0, v1, v2, v3, v4, v5, v6, v7, x, y, z), tri_mix(v8, v9, v10, v11, v12, v13, v14, v15, x, y, z),
+                w);
+}
+// (device/utils/noise.hpp:313-353): the 16 corners of the unit tesseract,
+// x fastest, then y, z and w
+RM_HD float perlin_4d(float x, float y, float z, float w) {
+    int X, Y, Z, W;
+    const float fx = floorfrac(x, X), fy = floorfrac(y, Y), fz = floorfrac(z, Z), fw = floorfrac(w, W);
+    const float u = fade(fx), v = fade(fy), t = fade(fz), s = fade(fw);
+    auto G = [&](int dx, int dy, int dz, int dw) {
+        return grad4((int)hash_uint4((uint32_t)(X + dx), (uint32_t)(Y + dy), (uint32_t)(Z + dz), (uint32_t)(W + dw)),
+                     fx - (float)dx, fy - (float)dy, fz - (float)dz, fw - (float)dw);
+    };
+    return quad_mix(G(0, 0, 0, 0), G(1, 0, 0, 0), G(0, 1, 0, 0), G(1, 1, 0, 0), G(0, 0, 1, 0), G(1, 0, 1, 0),
+                    G(0, 1, 1, 0), G(1, 1, 1, 0), G(0, 0, 0, 1), G(1, 0, 0, 1), G(0, 1, 0, 1), G(1, 1, 0, 1),
+                    G(0, 0, 1, 1), G(1, 0, 1, 1), G(0, 1, 1, 1), G(1, 1, 1, 1), u, v, t, s);
+}
+
 // IEEE-only finiteness test (device/utils/noise.hpp:387-392)
 RM_HD bool isfinite_safe(float f) {
     const uint32_t x = f2u(f);
@@ -152,13 +174,16 @@ RM_HD float ensure_finite(float v) { return isfinite_safe(v) ? v : 0.0f; }
 RM_HD float snoise1(float p) { return 0.2500f * ensure_finite(perlin_1d(p)); }
 RM_HD float snoise2(vec2 p) { return 0.6616f * ensure_finite(perlin_2d(p.x, p.y)); }
 RM_HD float snoise3(vec3 p) { return 0.9820f * ensure_finite(perlin_3d(p.x, p.y, p.z)); }
+RM_HD float snoise4(vec4 p) { return 0.8344f * ensure_finite(perlin_4d(p.x, p.y, p.z, p.w)); }
 RM_HD float noise1(float p) { return 0.5f * snoise1(p) + 0.5f; }
 RM_HD float noise2(vec2 p) { return 0.5f * snoise2(p) + 0.5f; }
 RM_HD float noise3(vec3 p) { return 0.5f * snoise3(p) + 0.5f; }
+RM_HD float noise4(vec4 p) { return 0.5f * snoise4(p) + 0.5f; }
 
 RM_HD float noise_any(float p) { return noise1(p); }
 RM_HD float noise_any(vec2 p) { return noise2(p); }
 RM_HD float noise_any(vec3 p) { return noise3(p); }
+RM_HD float noise_any(vec4 p) { return noise4(p); }
 
 // fBm with fractional octave blending (device/utils/noise.hpp:419-441)
 template <typename T>

@@ -143,6 +143,7 @@ struct rm_renderer {
     rm::ShadowQueue shadows{};
     rm_uniforms u{};
     uint32_t frame_index = 0;  // samples accumulated so far
+    uint32_t last_samples = 0; // samples of the last rm_render call (the image's content when not accumulating)
     std::vector<hipEvent_t> events;
     // radiance cache (rm_renderer_set_cache)
     rm_cache* cache = nullptr;

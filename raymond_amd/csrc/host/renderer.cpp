@@ -69,12 +69,17 @@ void renderer_init(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, size
     renderer_reset(r);
 }
 
+static void join_cache_work(rm_renderer* r);
+
 void renderer_reset(rm_renderer* r) {
     hipStream_t st = r->scene->dev->stream;
+    // pending cache lookups of the last stream add into the image
+    join_cache_work(r);
     RM_HIP(hipMemsetAsync(r->image_buf.ptr, 0, r->image_buf.bytes, st));
     if (r->image64_buf.ptr) RM_HIP(hipMemsetAsync(r->image64_buf.ptr, 0, r->image64_buf.bytes, st));
     RM_HIP(hipStreamSynchronize(st));
     r->frame_index = 0;
+    r->last_samples = 0;
 }
 
 namespace {
@@ -247,7 +252,6 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     const uint32_t total = br ? br->total : W * H * samples;
 
     StageTimer tm{r->u.timing != 0, st, {}, {}, {}, {}, r->events};
-    if (!r->u.accumulate) RM_HIP(hipMemsetAsync(image, 0, r->image_buf.bytes, st));
     RM_HIP(launch_clear_counters(cnt, st));
 
     FrameUniforms fu{};
@@ -586,6 +590,14 @@ void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
     RM_REQUIRE(s->envmap_built, "envmap not built");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const auto t0 = std::chrono::steady_clock::now();
+    if (!r->u.accumulate) {
+        // the image holds this call's samples only.  The cache work of earlier
+        // streams adds into it on the overlap stream, so it is joined before
+        // the one clear of the call; the streams of this call then all add.
+        join_cache_work(r);
+        RM_HIP(hipMemsetAsync(r->image_buf.ptr, 0, r->image_buf.bytes, s->dev->stream));
+    }
+    r->last_samples = samples;
     // bound a stream so its iteration count stays within kMaxIters.  With a
     // radiance cache, streams double in length from one sample to
     // kCacheStreamMax as the cache sees more samples: the cache learns
@@ -629,7 +641,11 @@ void renderer_read_image_f64(rm_renderer* r, double* rgba, size_t count) {
 void renderer_resolve(rm_renderer* r, float* rgba, size_t count) {
     RM_REQUIRE(count >= (size_t)r->width * r->height * 4, "output buffer too small");
     hipStream_t st = r->scene->dev->stream;
-    const float divisor = r->u.accumulate ? (float)(r->frame_index > 0 ? r->frame_index : 1) : 1.0f;
+    // accumulating: the samples so far (normalizeImage.metal:45-46 divides by
+    // frameIndex + 1, one sample per frame); otherwise the last call's samples,
+    // which is 1 for the reference's one-sample frames
+    const uint32_t n = r->u.accumulate ? r->frame_index : r->last_samples;
+    const float divisor = (float)(n > 0 ? n : 1);
     RM_HIP(launch_resolve(r->image_buf.as<float>(), r->resolve_buf.as<float>(), r->width * r->height, r->u.exposure,
                           divisor, (int)r->u.tonemapping, st));
     RM_HIP(hipMemcpyAsync(rgba, r->resolve_buf.ptr, (size_t)r->width * r->height * 16, hipMemcpyDeviceToHost, st));

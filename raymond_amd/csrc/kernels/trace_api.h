@@ -12,8 +12,14 @@ constexpr int kTraceBlock = 64;      // one wave64 per workgroup
 constexpr int kMaxTraceStack = 64;   // LDS rows + spill rows; BVH depth is bounded below this
 // Stack rows per lane in LDS.  The stack holds at most one entry per tree
 // level, so 12 rows cover BVHs up to depth 12 without spilling; 6 KB of LDS
-// per wave leaves the VGPRs, not LDS, limiting occupancy.
-constexpr int kLdsStack = 12;
+// per wave leaves the VGPRs, not LDS, limiting occupancy.  RM_LDS_STACK
+// changes the row count for a test build of the library (1 row: every
+// traversal below the root's children runs through the spill rows).
+#ifndef RM_LDS_STACK
+#define RM_LDS_STACK 12
+#endif
+static_assert(RM_LDS_STACK >= 1 && RM_LDS_STACK < 64, "LDS stack rows out of range");
+constexpr int kLdsStack = RM_LDS_STACK;
 struct ClosestArgs {
     const Bvh8Node* nodes;
     const BvhTri* tris;

@@ -788,7 +788,9 @@ def count_emitted_nodes(source):
 
 def bundled_material_sources(big=False):
     """(name, material source) of the scenes the tests, the smoke check and the
-    default benchmark build, for warming the code-object cache ahead of time."""
+    benchmark workloads build (the 32-graph scene as the benchmark's inlined
+    module; `big` adds its out-of-line form), for warming the code-object
+    cache ahead of time."""
     out = {}
     for name, desc in [("cornell", cornell_box(64, 64)),
                        ("small spheres", sphere_field(96, 54, grid=2, level=3)),
@@ -796,10 +798,10 @@ def bundled_material_sources(big=False):
                        ("lamp box", lamp_box(48, 48)),
                        ("sphere field", sphere_field(1920, 1080))]:
         out[name] = desc.material_source()
+    g = shader_graph_scene(1920, 1080)
+    out["graphs, inlined"] = g.material_source(False)
     if big:
-        g = shader_graph_scene(1920, 1080)
         out["graphs, out of line"] = g.material_source(True)
-        out["graphs, inlined"] = g.material_source(False)
     # scenes with equal materials share one module
     seen, jobs = set(), []
     for name, src in out.items():

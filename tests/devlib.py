@@ -14,27 +14,55 @@ SRC = os.path.join(HERE, "native", "devlib_shim.cpp")
 BUILD = os.path.join(HERE, "native", "_build")
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-shared", "-Wno-unused-function"]
 
-_LIB = None
+_LIBS = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is not None:
-        return _LIB
+def lib(plant=None):
+    """The shim over the device headers.  plant = (header, old, new): a build
+    over a copy of the headers with `old` replaced by `new` in `header` (a
+    mistake planted to show that a check catches it); `old` must occur."""
+    key = plant
+    if key in _LIBS:
+        return _LIBS[key]
+    texts = {}
+    for f in sorted(os.listdir(DEVICE)):
+        if f.endswith(".h"):
+            with open(os.path.join(DEVICE, f)) as fh:
+                texts[f] = fh.read()
+    if plant is not None:
+        header, old, new = plant
+        assert old in texts[header], "planted mistake does not apply: %r" % (plant,)
+        texts[header] = texts[header].replace(old, new)
     h = hashlib.sha1()
-    for p in [SRC] + sorted(os.path.join(DEVICE, f) for f in os.listdir(DEVICE) if f.endswith(".h")):
-        with open(p, "rb") as f:
-            h.update(f.read())
+    with open(SRC, "rb") as fh:
+        h.update(fh.read())
+    for f, t in texts.items():
+        h.update(f.encode())
+        h.update(t.encode())
     os.makedirs(BUILD, exist_ok=True)
-    so = os.path.join(BUILD, "devlib_%s.so" % h.hexdigest()[:12])
+    tag = h.hexdigest()[:12]
+    so = os.path.join(BUILD, "devlib_%s.so" % tag)
     if not os.path.exists(so):
+        inc = DEVICE
+        if plant is not None:
+            inc = os.path.join(BUILD, "planted_%s" % tag)
+            os.makedirs(inc, exist_ok=True)
+            for f, t in texts.items():
+                with open(os.path.join(inc, f), "w") as fh:
+                    fh.write(t)
         tmp = so + ".tmp%d" % os.getpid()
-        subprocess.run(["g++"] + FLAGS + ["-I", DEVICE, "-o", tmp, SRC], check=True)
+        subprocess.run(["g++"] + FLAGS + ["-I", inc, "-o", tmp, SRC], check=True)
         os.replace(tmp, so)
-    _LIB = C.CDLL(so)
-    _LIB.shim_hash_uint3.restype = C.c_uint32
-    _LIB.shim_hash_uint3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
-    return _LIB
+    L = C.CDLL(so)
+    for name, args in (("shim_hash_uint", [C.c_uint32]), ("shim_hash_uint2", [C.c_uint32] * 2),
+                       ("shim_hash_uint3", [C.c_uint32] * 3), ("shim_hash_uint4", [C.c_uint32] * 4)):
+        getattr(L, name).restype = C.c_uint32
+        getattr(L, name).argtypes = args
+    L.shim_shading_context.restype = C.c_uint32
+    L.shim_fractal4.restype = C.c_float
+    L.shim_fractal4.argtypes = [C.c_void_p, C.c_float, C.c_float]
+    _LIBS[key] = L
+    return L
 
 
 def f32(a):
@@ -153,4 +181,152 @@ def hsv_roundtrip(rgb):
     rgb = f32(rgb)
     out = np.zeros_like(rgb)
     lib().shim_hsv(C.c_void_p(rgb.ctypes.data), C.c_void_p(out.ctypes.data), rgb.shape[0])
+    return out
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def lobe_eval(kind, params, wo, wi, L=None):
+    """One lobe (1 Specular, 2 Transmission, 3 Clearcoat) from explicit
+    parameters: value (n, 3) and pdf (n,) of wo -> wi in the shading frame."""
+    L = L or lib()
+    p, wo, wi = f32(params), f32(wo), f32(wi)
+    n = wo.shape[0]
+    val = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    L.shim_lobe_eval(kind, _p(p), _p(wo), _p(wi), _p(val), _p(pdf), n)
+    return val, pdf
+
+
+def lobe_sample(kind, params, wo, rnd, L=None):
+    L = L or lib()
+    p, wo, rnd = f32(params), f32(wo), f32(rnd)
+    n = wo.shape[0]
+    wi = np.zeros((n, 3), np.float32)
+    w = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    L.shim_lobe_sample(kind, _p(p), _p(wo), _p(rnd), _p(wi), _p(w), _p(pdf), n)
+    return wi, w, pdf
+
+
+PRINCIPLED_IN = ["base_r", "base_g", "base_b", "metallic", "roughness", "specular", "specular_tint", "sheen",
+                 "sheen_tint", "clearcoat", "clearcoat_roughness", "transmission", "ior", "anisotropic",
+                 "transmission_roughness", "alpha", "emission_r", "emission_g", "emission_b", "emission_strength"]
+
+
+def principled(**kw):
+    """BsdfPrincipled::compute; returns a dict of the UberShader fields."""
+    L = kw.pop("L", None) or lib()
+    d = dict(base_r=0.8, base_g=0.8, base_b=0.8, metallic=0.0, roughness=0.5, specular=0.5, specular_tint=0.0,
+             sheen=0.0, sheen_tint=0.5, clearcoat=0.0, clearcoat_roughness=0.03, transmission=0.0, ior=1.45,
+             anisotropic=0.0, transmission_roughness=0.0, alpha=1.0, emission_r=0.0, emission_g=0.0, emission_b=0.0,
+             emission_strength=0.0)
+    for k, v in kw.items():
+        assert k in d, k
+        d[k] = v
+    x = f32([d[k] for k in PRINCIPLED_IN])
+    o = np.zeros(34, np.float32)
+    L.shim_principled(_p(x), _p(o))
+    o = o.astype(np.float64)
+    return dict(lobe_prob=o[0:4], diffuse_weight=o[4:7], sheen_weight=o[7:10], diffuse_roughness=o[10],
+                alpha_x=o[11], alpha_y=o[12], cspec0=o[13:16], specular_ior=o[16], specular_weight=o[17],
+                reflection_alpha=o[18], transmission_alpha=o[19], transmission_base=o[20:23],
+                transmission_cspec0=o[23:26], transmission_ior=o[26], transmission_weight=o[27],
+                clearcoat_alpha=o[28], clearcoat_weight=o[29], alpha=o[30], emission=o[31:34])
+
+
+def shading_context(verts, normals, uvs, faces, mats, m, nm, bounds, prim, b0, b1, L=None):
+    """ShadingContext::build; returns (dict of vectors, shader index)."""
+    L = L or lib()
+    verts, normals, uvs = f32(verts), f32(normals), f32(uvs)
+    faces = u32(faces)
+    mats = np.ascontiguousarray(mats, np.uint16)
+    m = f32(np.asarray(m, np.float64).T.reshape(-1))      # column-major
+    nm = f32(np.asarray(nm, np.float64).T.reshape(-1))
+    bounds = f32(bounds)
+    out = np.zeros(24, np.float32)
+    idx = L.shim_shading_context(_p(verts), _p(normals), _p(uvs), _p(faces), _p(mats), _p(m), _p(nm), _p(bounds),
+                                 C.c_uint32(prim), C.c_float(b0), C.c_float(b1), _p(out))
+    names = ["uv", "position", "generated", "object", "normal", "true_normal", "tu", "tv"]
+    return {k: out[3 * i:3 * i + 3].astype(np.float64) for i, k in enumerate(names)}, int(idx)
+
+
+def shape_light(verts, faces, cdf, area, m, nm, origin, seeds_a, seeds_b, L=None):
+    """ShapeLight::sample from `origin`: (n, 7) = direction (3), distance,
+    pdf, weight, pdf recomputed from the sample's shading context."""
+    L = L or lib()
+    verts = f32(verts)
+    normals = f32(np.zeros_like(verts))
+    uvs = f32(np.zeros((verts.shape[0], 2)))
+    faces = u32(faces)
+    cdf = f32(cdf)
+    m = f32(np.asarray(m, np.float64).T.reshape(-1))
+    nm = f32(np.asarray(nm, np.float64).T.reshape(-1))
+    origin = f32(origin)
+    sa, sb = u32(seeds_a), u32(seeds_b)
+    n = sa.size
+    out = np.zeros((n, 7), np.float32)
+    L.shim_shape_light(_p(verts), _p(normals), _p(uvs), _p(faces), faces.shape[0], _p(cdf), C.c_float(area), _p(m),
+                       _p(nm), _p(origin), _p(sa), _p(sb), n, _p(out))
+    return out
+
+
+def perlin(dim, p, L=None):
+    """Signed, unscaled Perlin noise of dimension 1-4 at points p (n, dim)."""
+    L = L or lib()
+    p = np.asarray(p, np.float32).reshape(-1, dim)
+    q = np.zeros((p.shape[0], 4), np.float32)
+    q[:, :dim] = p
+    out = np.zeros(p.shape[0], np.float32)
+    L.shim_perlin(dim, _p(q), _p(out), p.shape[0])
+    return out
+
+
+def fractal4(p, octaves, roughness, L=None):
+    L = L or lib()
+    q = f32(p)
+    return float(L.shim_fractal4(_p(q), octaves, roughness))
+
+
+def ramp4(pos, colors, fac, L=None):
+    L = L or lib()
+    pos, colors, fac = f32(pos), f32(colors), f32(fac)
+    out = np.zeros((fac.size, 4), np.float32)
+    L.shim_ramp4(_p(pos), _p(colors), _p(fac), _p(out), fac.size)
+    return out
+
+
+MATH_OPS = ["ADD", "SUBTRACT", "MULTIPLY", "DIVIDE", "MULTIPLY_ADD", "POWER", "MINIMUM", "MAXIMUM", "TANGENT",
+            "LESS_THAN", "GREATER_THAN", "MODULO"]
+
+
+def math_node(op, a, b, c=0.0, clamp=False, L=None):
+    L = L or lib()
+    a = f32(np.atleast_1d(a))
+    b = f32(np.broadcast_to(b, a.shape))
+    c = f32(np.broadcast_to(c, a.shape))
+    out = np.zeros(a.size, np.float32)
+    L.shim_math(MATH_OPS.index(op), int(clamp), _p(a), _p(b), _p(c), _p(out), a.size)
+    return out
+
+
+BLENDS = ["MIX", "ADD", "MULTIPLY", "SCREEN", "OVERLAY", "SUB", "COLOR", "LIGHTEN", "DARKEN", "VALUE"]
+
+
+def mix_node(blend, c1, c2, fac, clamp=False, L=None):
+    L = L or lib()
+    c1, c2 = f32(c1).reshape(-1, 4), f32(c2).reshape(-1, 4)
+    fac = f32(np.broadcast_to(fac, (c1.shape[0],)))
+    out = np.zeros_like(c1)
+    L.shim_mix(BLENDS.index(blend), int(clamp), _p(c1), _p(c2), _p(fac), _p(out), c1.shape[0])
+    return out
+
+
+def mix_shader(fac, e0, e1, rnd, L=None):
+    L = L or lib()
+    e0, e1, rnd = f32(e0), f32(e1), f32(rnd)
+    out = np.zeros((rnd.size, 5), np.float32)
+    L.shim_mix_shader(C.c_float(fac), _p(e0), _p(e1), _p(rnd), _p(out), rnd.size)
     return out

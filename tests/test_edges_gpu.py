@@ -7,6 +7,7 @@ import pytest
 import raymond_amd.api as api
 import raymond_amd.scenes as scenes
 from oracle import binding
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 
@@ -35,18 +36,19 @@ def per_sample(scene, orc, W, H, depth, frames=2, spl=1):
     """Each sample of the GPU stream against the oracle's frame: ray counts
     per depth equal, radiance within 1e-4 of the frame's peak."""
     r = api.Renderer(scene, W, H, max_depth=depth, samples_per_launch=spl)
-    prev = np.zeros((H, W, 4), np.float32)
+    r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
+    total = np.zeros((H, W, 4), np.float32)
     for frame in range(frames):
         st = r.execute(1)
         img = r.image()
-        sample = img - prev
-        prev = img
+        sample = img
+        total += img
         ref, res = orc.render(W, H, depth, frames=1, frame_begin=frame)
         n = min(depth, 32)
         assert [st.rays[d] for d in range(n)] == [res.rays[d] for d in range(n)]
         assert [st.shadow_rays[d] for d in range(n)] == [res.shadow_rays[d] for d in range(n)]
-        assert np.abs(sample - ref).max() <= 1e-4 * max(np.abs(ref).max(), 1e-6) + 1e-6
-    return prev
+        parity.assert_radiance_close(sample, ref)
+    return total
 
 
 def test_scene_without_geometry_renders_the_world(device):

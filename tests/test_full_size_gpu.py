@@ -1,0 +1,335 @@
+"""GPU parity at the benchmarked sizes, against the CPU oracle.
+
+* Sphere field (5,242,882 triangles, 1920x1080, depth 12): closest hits of a
+  full frame of camera rays and 1M first-bounce rays bit-exact against the
+  oracle's own BVH2 (and a brute-force sample), 1M any-hit rays exact, the
+  2048^2 importance map bit-exact, per-sample radiance on a fixed 4096-pixel
+  subset within 1e-4 per pixel.
+* Shader graphs: all 32 random graphs of the workload, compiled into the
+  inlined module the benchmark runs, material-sorted, per-sample on a subset.
+* Batch: uneven jobs including a 1.2M-triangle one, each job against the
+  oracle's render of the same samples.
+* Radiance cache: the default 2^19-entry table against the FP64 oracle, and a
+  1080p render with the cache.
+* Converged image: the Cornell box accumulated in FP64 on the GPU and in the
+  oracle, with their RMSE against a 16k-sample FP64 reference.
+* Traversal-stack spill rows (the library built with one LDS row) against
+  brute force, and the image clear of non-accumulating cached renders.
+
+Frames are compared one at a time with accumulation off (each rm_render
+call's image holds that call's samples only), never by differencing sums.
+"""
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from oracle import binding
+from tests import parity
+from tests.test_rcache_cpu import random_features, random_params
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+T = parity.ORACLE_THREADS
+SUBSET = 4096
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Sphere field (the default benchmark workload)
+
+@pytest.fixture(scope="module")
+def field(device):
+    t0 = time.time()
+    desc = scenes.sphere_field()
+    scene = scenes.build_device_scene(device, desc)
+    st = scene.stats()
+    log("sphere field: %d triangles, BVH8 %d nodes depth %d, GPU scene %.1f s"
+        % (st.triangles, st.bvh_nodes, st.bvh_depth, time.time() - t0))
+    t0 = time.time()
+    orc = binding.OracleScene(desc)
+    orc.build_envmap(threads=T)
+    log("oracle scene + 2048^2 envmap: %.1f s on %d threads" % (time.time() - t0, T))
+    return desc, scene, orc
+
+
+def test_field_is_the_benchmarked_scene(field):
+    desc, scene, orc = field
+    st = scene.stats()
+    assert st.triangles == 5242882
+    assert (desc.width, desc.height, desc.envmap_resolution) == (1920, 1080, 2048)
+
+
+def test_field_envmap_bit_exact(field):
+    desc, scene, orc = field
+    gp, gm = scene.read_envmap()
+    op, om = orc.read_envmap()
+    assert gp.size == 2048 * 2048
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+    assert np.array_equal(gm.view(np.uint32), om.view(np.uint32))
+
+
+def test_field_hits_bit_exact(field):
+    """Every pixel's camera ray and 1,048,576 first-bounce rays: t, instance,
+    primitive and barycentrics equal bit for bit (GPU BVH8 against the
+    oracle's BVH2); any hit on 1,048,576 bounded rays equal."""
+    desc, scene, orc = field
+    cam = parity.camera_rays(desc, 1920, 1080, seed=21)
+    t0 = time.time()
+    g = scene.trace_closest(cam)
+    r = orc.trace_closest(cam, threads=T)
+    log("camera rays: %d, hit %.3f, oracle %.1f s" % (cam.shape[0], np.mean(r[0] >= 0), time.time() - t0))
+    parity.assert_hits_equal(g, r)
+    t_cam = r[0]
+    bounce = parity.bounce_rays(cam, t_cam, seed=22, count=1 << 20)
+    g = scene.trace_closest(bounce)
+    r = orc.trace_closest(bounce, threads=T)
+    log("bounce rays: %d, hit %.3f" % (bounce.shape[0], np.mean(r[0] >= 0)))
+    parity.assert_hits_equal(g, r)
+    assert np.mean(r[0] >= 0) > 0.3
+    rng = np.random.default_rng(23)
+    shadow = parity.bounce_rays(cam, t_cam, seed=24, count=1 << 20)
+    shadow[:, 7] = rng.uniform(0.05, 3.0, shadow.shape[0])
+    occ_g = scene.trace_any(shadow)
+    occ_r = orc.trace_any(shadow, threads=T)
+    log("any-hit rays: %d, occluded %.3f" % (shadow.shape[0], np.mean(occ_r)))
+    assert np.array_equal(occ_g, occ_r)
+    assert 0.05 < np.mean(occ_r) < 0.95
+
+
+def test_field_bvh2_matches_brute_force_sample(field):
+    """The oracle's BVH2 on this scene against the exhaustive loop (512 rays,
+    2.7e9 triangle tests)."""
+    desc, scene, orc = field
+    cam = parity.camera_rays(desc, 1920, 1080, seed=25, count=512)
+    parity.assert_hits_equal(orc.trace_closest(cam, threads=T), orc.trace_closest(cam, brute_force=True, threads=T))
+
+
+def render_frames_subset(scene, orc, W, H, depth, frames, pixels, **uniforms):
+    """GPU frames (accumulation off: each call's image is that frame) against
+    the oracle's render of the same frame on `pixels`; returns the worst
+    relative error."""
+    r = api.Renderer(scene, W, H, max_depth=depth)
+    r.set_uniforms(accumulate=0, **uniforms)
+    worst = 0.0
+    for f in range(frames):
+        r.execute(1)
+        img = r.image()
+        t0 = time.time()
+        ref, res = orc.render(W, H, depth, frames=1, frame_begin=f, pixels=pixels, threads=T,
+                              rr_depth=uniforms.get("rr_depth", 0))
+        assert res.paths == pixels.size
+        w = parity.assert_radiance_close(parity.subset_of(img, pixels), parity.subset_of(ref, pixels),
+                                         what="frame %d" % f)
+        log("frame %d: %d pixels, worst relative error %.3g, oracle %.1f s" % (f, pixels.size, w, time.time() - t0))
+        worst = max(worst, w)
+        assert parity.subset_of(ref, pixels)[:, 3].sum() > 0
+    return worst
+
+
+def test_field_per_sample_subset(field):
+    desc, scene, orc = field
+    px = parity.pixel_subset(1920, 1080, SUBSET, seed=26)
+    render_frames_subset(scene, orc, 1920, 1080, 12, 2, px, material_sort=1)
+
+
+# ---------------------------------------------------------------------------
+# Shader graphs: the 32-graph module of the benchmark
+
+@pytest.fixture(scope="module")
+def graphs32(device):
+    desc = scenes.shader_graph_scene(1920, 1080)
+    assert len(desc.surface_materials) == 33
+    t0 = time.time()
+    scene = scenes.build_device_scene(device, desc, noinline=False)
+    log("32-graph scene, inlined module: %.1f s (JIT included unless cached)" % (time.time() - t0))
+    orc = binding.OracleScene(desc, desc.material_source(False))
+    orc.build_envmap(threads=T)
+    return desc, scene, orc
+
+
+@pytest.mark.timeout(900)
+def test_graphs32_per_sample_subset(graphs32):
+    desc, scene, orc = graphs32
+    px = parity.pixel_subset(1920, 1080, SUBSET, seed=27)
+    render_frames_subset(scene, orc, 1920, 1080, 8, 2, px, material_sort=1)
+
+
+# ---------------------------------------------------------------------------
+# Batch with a job of more than a million triangles
+
+def test_batch_jobs_match_oracle(device):
+    """Jobs of 1.2M, 50k and 1k triangles at different sizes, spp and depths
+    rendered as one batch; each job's image (the sum of its samples) equals
+    the oracle's render of the same samples on a pixel subset."""
+    specs = [(1_200_000, 512, 512, 3, 8), (50_000, 768, 768, 2, 12), (1_000, 256, 256, 4, 6)]
+    jobs, refs = [], []
+    for k, (tris, w, h, spp, depth) in enumerate(specs):
+        d = scenes.tessellated_cornell_box(w, h, tris)
+        d.envmap_resolution = 64
+        s = scenes.build_device_scene(device, d)
+        log("job %d: %d triangles, %dx%d, %d spp, depth %d" % (k, s.stats().triangles, w, h, spp, depth))
+        jobs.append((s, w, h, spp, depth, 0))
+        orc = binding.OracleScene(d)
+        orc.build_envmap(threads=T)
+        px = parity.pixel_subset(w, h, 2048, seed=30 + k)
+        ref, _ = orc.render(w, h, depth, frames=spp, pixels=px, threads=T)
+        refs.append((px, ref))
+        orc.close()
+    assert jobs[0][0].stats().triangles >= 1_000_000
+    b = api.Batch(device, jobs)
+    st = b.render(use_graph=1)
+    assert st.samples == sum(w * h * spp for (_, w, h, spp, _, _) in jobs)
+    for j, (px, ref) in enumerate(refs):
+        w = parity.assert_radiance_close(parity.subset_of(b.image(j), px), parity.subset_of(ref, px),
+                                         what="job %d" % j)
+        log("job %d: worst relative error %.3g" % (j, w))
+
+
+# ---------------------------------------------------------------------------
+# Radiance cache with the default table
+
+def test_cache_default_table_matches_oracle(device):
+    log2 = 19
+    c = api.RadianceCache(device, log2_table_size=log2, seed=1)
+    assert c.table_floats == 16 * 2 * (1 << log2)
+    rng = np.random.default_rng(40)
+    mlp, table = random_params(rng, log2)
+    c.write_params(mlp, table)
+    f = random_features(rng, 4096)
+    got = c.infer(f)
+    ref = binding.rc_forward(log2, mlp, table, f)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    log("2^19 table forward: relative L2 %.3g" % rel)
+    assert rel < 1e-2
+    y = ref
+    t = (y + rng.normal(size=y.shape) * 0.5).astype(np.float32)
+    gm, gt, loss = c.gradients(f[:512], t[:512])
+    rm, rt, rloss = binding.rc_gradients(log2, mlp, table, f[:512], t[:512])
+    assert abs(loss - rloss) < 1e-2 * rloss
+    for name, sl in (("W1", slice(0, 3072)), ("W2", slice(3072, 7168)), ("W3", slice(7168, 9216))):
+        e = np.linalg.norm(gm[sl] - rm[sl]) / np.linalg.norm(rm[sl])
+        log("2^19 table %s gradient: relative L2 %.3g" % (name, e))
+        assert e < 3e-2, name
+    e = np.linalg.norm(gt - rt) / np.linalg.norm(rt)
+    log("2^19 table gradient: relative L2 %.3g" % e)
+    assert e < 3e-2
+
+
+def test_cache_render_1080p(device):
+    """The cache inside a 1080p Cornell render with the default table: paths
+    end at their lookups (fewer rays), every lookup is answered, the image is
+    finite and its mean is within 10% of the plain render's."""
+    desc = scenes.cornell_box(1920, 1080)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc)
+    plain = api.Renderer(scene, 1920, 1080, max_depth=12)
+    plain.set_uniforms(rr_depth=3)
+    sp = plain.execute(8)
+    c = api.RadianceCache(device, seed=7)
+    r = api.Renderer(scene, 1920, 1080, max_depth=12)
+    r.set_uniforms(rr_depth=3)
+    r.set_cache(c, mode=1)
+    sc = r.execute(8)
+    cs = c.stats()
+    img, ref = r.image(), plain.image()
+    log("1080p cache: %d lookups, %d records, %d steps, rays %d vs plain %d"
+        % (cs.queries, cs.train_records, cs.steps, sc.total_rays, sp.total_rays))
+    assert np.all(np.isfinite(img))
+    assert cs.queries > 1920 * 1080 and cs.train_records > 0
+    assert sc.total_rays < 0.8 * sp.total_rays
+    m, mp = img[..., :3].mean(), ref[..., :3].mean()
+    assert abs(m - mp) < 0.1 * mp, (m, mp)
+
+
+def test_cached_frames_without_accumulation(device):
+    """Non-accumulating renders with a cache (whose lookups are added to the
+    image on a second stream, overlapping the next stream): each call's image
+    holds exactly that call's samples, equal to the difference of an
+    accumulating renderer's images."""
+    desc = scenes.cornell_box(64, 64)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc)
+    c = api.RadianceCache(device, seed=8, batch_size=1024)
+    tr = api.Renderer(scene, 64, 64, max_depth=12)
+    tr.set_uniforms(rr_depth=3)
+    tr.set_cache(c, mode=1)
+    tr.execute(16)
+    out = []
+    for accumulate in (0, 1):
+        r = api.Renderer(scene, 64, 64, max_depth=12)
+        r.set_uniforms(rr_depth=3, accumulate=accumulate)
+        r.set_cache(c, mode=2)  # query only: the parameters stay fixed
+        r.execute(2)
+        first = r.image()
+        r.execute(6)  # several streams (they grow with the samples seen)
+        out.append(r.image() - (0 if accumulate == 0 else first))
+    frames, diff = out
+    assert frames[..., 3].sum() > 0
+    scale = np.abs(diff).max()
+    assert np.abs(frames - diff).max() <= 1e-5 * scale
+
+
+# ---------------------------------------------------------------------------
+# Converged image: FP64 accumulation on both sides
+
+def test_converged_cornell_fp64(device):
+    """The Cornell box (roulette from bounce 3) at 128x128, 256 samples: the
+    GPU's FP64 image and the oracle's FP64 image of the same samples agree to
+    1e-6 relative RMSE, and both have the same RMSE against a 16384-sample
+    FP64 reference."""
+    W = H = 128
+    N = 256
+    desc = scenes.cornell_box(W, H)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc)
+    r = api.Renderer(scene, W, H, max_depth=8)
+    r.set_uniforms(rr_depth=3, accumulate_f64=1)
+    for _ in range(N):  # one sample per stream: each sample is summed in FP32, the samples in FP64
+        r.execute(1)
+    gpu = r.image_f64()[..., :3] / N
+    orc = binding.OracleScene(desc)
+    orc.build_envmap(threads=T)
+    t0 = time.time()
+    o64, _ = orc.render_f64(W, H, 8, frames=N, rr_depth=3, threads=T)
+    log("oracle FP64 %d spp: %.1f s" % (N, time.time() - t0))
+    ora = o64[..., :3] / N
+    ref_r = api.Renderer(scene, W, H, max_depth=8, samples_per_launch=8)
+    ref_r.set_uniforms(rr_depth=3, accumulate_f64=1)
+    ref_r.execute(16384)
+    ref = ref_r.image_f64()[..., :3] / 16384.0
+    mean = ref.mean()
+    rmse_go = np.sqrt(np.mean((gpu - ora) ** 2))
+    rmse_g = np.sqrt(np.mean((gpu - ref) ** 2))
+    rmse_o = np.sqrt(np.mean((ora - ref) ** 2))
+    log("converged Cornell 128^2 x %d spp: RMSE(gpu, oracle) %.3g, RMSE vs 16k reference: gpu %.6g oracle %.6g "
+        "(mean radiance %.4g)" % (N, rmse_go, rmse_g, rmse_o, mean))
+    assert rmse_go <= 1e-6 * mean
+    assert abs(rmse_g - rmse_o) <= 1e-4 * rmse_o
+    assert rmse_g < 0.1 * mean
+
+
+# ---------------------------------------------------------------------------
+# Traversal-stack spill rows
+
+def test_traversal_spill_rows():
+    lib = os.path.join(ROOT, "raymond_amd", "libraymond_amd_spill.so")
+    assert os.path.exists(lib), "spill test library not built (raymond_amd/build.py)"
+    env = dict(os.environ, RM_LIBRARY=lib)
+    p = subprocess.run([sys.executable, "-m", "tests.spill_check"], cwd=ROOT, env=env, capture_output=True,
+                       text=True, timeout=600)
+    log(p.stdout[-2000:])
+    assert p.returncode == 0, p.stderr[-4000:]
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    for name, v in res.items():
+        assert v["bvh_depth"] >= 3, (name, v)

@@ -1,11 +1,13 @@
 """GPU parity against the CPU oracle: traversal hits bit-exact, per-sample
-radiance within 1e-4 relative, identical per-depth ray counts."""
+radiance within 1e-4 relative per pixel (tests/parity.py), identical per-depth
+ray counts."""
 import numpy as np
 import pytest
 
 import raymond_amd.api as api
 import raymond_amd.scenes as scenes
 from oracle import binding
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 
@@ -52,15 +54,7 @@ def edge_rays(desc, n, seed):
     return r
 
 
-def assert_hits_equal(gpu, ref):
-    t, inst, prim, bary = gpu
-    rt, rinst, rprim, rbary = ref
-    hit = rt >= 0
-    assert np.array_equal(t >= 0, hit), "hit/miss mismatch: %d rays" % np.sum((t >= 0) != hit)
-    assert np.array_equal(t.view(np.uint32)[hit], rt.view(np.uint32)[hit]), "distance bits differ"
-    assert np.array_equal(inst[hit], rinst[hit]), "instance ids differ"
-    assert np.array_equal(prim[hit], rprim[hit]), "primitive ids differ"
-    assert np.array_equal(bary.view(np.uint32)[hit], rbary.view(np.uint32)[hit]), "barycentric bits differ"
+assert_hits_equal = parity.assert_hits_equal
 
 
 @pytest.fixture(scope="module")
@@ -141,18 +135,18 @@ def test_render_cornell_per_sample(cornell):
     W, H = 64, 64
     r = api.Renderer(scene, W, H, max_depth=8)
     ref_total = np.zeros((H, W, 4), np.float32)
-    prev = np.zeros((H, W, 4), np.float32)
+    r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
+    total = np.zeros((H, W, 4), np.float32)
     for frame in range(3):
         st = r.execute(1)
         img = r.image()
-        sample = img - prev
-        prev = img
+        sample = img
+        total += img
         ref, res = orc.render(W, H, 8, frames=1, frame_begin=frame)
         ref_total += ref
         assert [st.rays[d] for d in range(8)] == [res.rays[d] for d in range(8)]
         assert [st.shadow_rays[d] for d in range(7)] == [res.shadow_rays[d] for d in range(7)]
-        scale = max(np.abs(ref).max(), 1e-6)
-        assert np.abs(sample - ref).max() <= 1e-4 * scale + 1e-6
+        parity.assert_radiance_close(sample, ref)
 
 
 def test_render_cornell_config_mode_rr3(cornell):
@@ -163,7 +157,7 @@ def test_render_cornell_config_mode_rr3(cornell):
     st = r.execute(1)
     ref, res = orc.render(W, H, 8, frames=1, rr_depth=3)
     assert st.total_rays == res.total_rays
-    assert np.abs(r.image() - ref).max() <= 1e-4 * max(np.abs(ref).max(), 1e-6)
+    parity.assert_radiance_close(r.image(), ref)
 
 
 @pytest.fixture(scope="module")
@@ -197,8 +191,7 @@ def test_render_spheres_per_sample(small_spheres):
     st = r.execute(1)
     ref, res = orc.render(W, H, 12, frames=1)
     assert st.total_rays == res.total_rays
-    scale = max(np.abs(ref).max(), 1e-6)
-    assert np.abs(r.image() - ref).max() <= 1e-4 * scale
+    parity.assert_radiance_close(r.image(), ref)
 
 
 @pytest.mark.parametrize("spl,samples,graph", [(1, 3, 0), (2, 5, 0), (2, 5, 1)])
@@ -214,8 +207,7 @@ def test_regenerating_stream_matches_frames(cornell, spl, samples, graph):
     ref, res = orc.render(W, H, 8, frames=samples, frame_begin=0)
     assert st.total_rays == res.total_rays
     assert st.iterations > 8
-    scale = max(np.abs(ref).max(), 1e-6)
-    assert np.abs(r.image() - ref).max() <= 1e-4 * scale
+    parity.assert_radiance_close(r.image(), ref)
 
 
 @pytest.fixture(scope="module")
@@ -235,17 +227,17 @@ def test_render_graphs_per_sample(graphs, sort):
     W, H = 48, 48
     r = api.Renderer(scene, W, H, max_depth=8)
     r.set_uniforms(material_sort=sort)
-    prev = np.zeros((H, W, 4), np.float32)
+    r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
+    total = np.zeros((H, W, 4), np.float32)
     for frame in range(2):
         st = r.execute(1)
         img = r.image()
-        sample = img - prev
-        prev = img
+        sample = img
+        total += img
         ref, res = orc.render(W, H, 8, frames=1, frame_begin=frame)
         assert [st.rays[d] for d in range(8)] == [res.rays[d] for d in range(8)]
         assert [st.shadow_rays[d] for d in range(7)] == [res.shadow_rays[d] for d in range(7)]
-        scale = max(np.abs(ref).max(), 1e-6)
-        assert np.abs(sample - ref).max() <= 1e-4 * scale + 1e-6
+        parity.assert_radiance_close(sample, ref)
 
 
 def test_material_sort_matches_unsorted(graphs):
@@ -262,8 +254,7 @@ def test_material_sort_matches_unsorted(graphs):
         out.append((st.total_rays, st.iterations, r.image()))
     assert out[0][0] == out[1][0]
     assert out[0][1] == out[1][1]
-    scale = max(np.abs(out[0][2]).max(), 1e-6)
-    assert np.abs(out[0][2] - out[1][2]).max() <= 1e-5 * scale
+    parity.assert_radiance_close(out[1][2], out[0][2], rtol=1e-5, what="sorted vs unsorted")
 
 
 def test_graph_replay_matches_direct_launches(cornell):
@@ -282,8 +273,7 @@ def test_graph_replay_matches_direct_launches(cornell):
             counts.append([st.rays[d] for d in range(33)] + [st.shadow_rays[d] for d in range(32)])
         out.append((counts, r.image()))
     assert out[0][0] == out[1][0]
-    scale = max(np.abs(out[0][1]).max(), 1e-6)
-    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-5 * scale
+    parity.assert_radiance_close(out[1][1], out[0][1], rtol=1e-5, what="graph replay vs direct")
 
 
 @pytest.fixture(scope="module")
@@ -306,17 +296,17 @@ def test_render_lamps_per_sample(lamps, mode):
     W, H = 48, 48
     r = api.Renderer(scene, W, H, max_depth=6)
     r.set_uniforms(sampling_mode=mode)
-    prev = np.zeros((H, W, 4), np.float32)
+    r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
+    total = np.zeros((H, W, 4), np.float32)
     for frame in range(2):
         st = r.execute(1)
         img = r.image()
-        sample = img - prev
-        prev = img
+        sample = img
+        total += img
         ref, res = orc.render(W, H, 6, frames=1, frame_begin=frame, sampling_mode=mode)
         assert [st.rays[d] for d in range(6)] == [res.rays[d] for d in range(6)]
         assert [st.shadow_rays[d] for d in range(5)] == [res.shadow_rays[d] for d in range(5)]
-        scale = max(np.abs(ref).max(), 1e-6)
-        assert np.abs(sample - ref).max() <= 1e-4 * scale + 1e-6
+        parity.assert_radiance_close(sample, ref)
         if mode != 0:
             assert ref.max() > 0.0
 

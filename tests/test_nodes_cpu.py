@@ -221,3 +221,84 @@ def test_shader_nodes_compile():
     img, _ = orc.render(4, 4, 1, frames=1)
     orc.close()
     check(img[..., :3].astype(np.float64), [1.0, 1.0, 1.0])  # seen by a camera ray
+
+
+# ---------------------------------------------------------------------------
+# The benchmark graphs' node kinds through the codegen, in closed form
+
+def test_color_ramp_through_codegen():
+    pos = [0.1, 0.35, 0.6, 0.9]
+    cols = [[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1], [1, 1, 1, 1]]
+    els = [{"position": p, "color": c} for p, c in zip(pos, cols)]
+    for fac in (0.05, 0.2, 0.5, 0.75, 0.95):
+        img = render_value({"N": node("VALTORGB", {"Fac": _in("VALUE", fac)},
+                                      {"color_mode": "RGB", "interpolation": "LINEAR", "elements": els})},
+                           ("N", "Color"))
+        check(img, [np.interp(fac, pos, [c[k] for c in cols]) for k in range(3)], rel=1e-5)
+
+
+@pytest.mark.parametrize("op,clamp,a,b,want", [
+    ("ADD", False, 0.3, 0.45, 0.75), ("ADD", True, 0.7, 0.6, 1.0), ("MULTIPLY", False, 0.6, 1.5, 0.9),
+    ("POWER", False, 0.5, 2.2, 0.5 ** 2.2), ("POWER", True, 1.5, 2.0, 1.0)])
+def test_math_through_codegen(op, clamp, a, b, want):
+    img = render_value({"N": node("MATH", {"Value": _in("VALUE", a), "Value_001": _in("VALUE", b),
+                                           "Value_002": _in("VALUE", 0.5)},
+                                  {"operation": op, "use_clamp": clamp})}, ("N", "Value"))
+    check(img, [want] * 3, rel=1e-5)
+
+
+@pytest.mark.parametrize("blend", ["MIX", "ADD", "MULTIPLY", "SCREEN", "OVERLAY", "LIGHTEN", "DARKEN"])
+def test_mix_rgb_through_codegen(blend):
+    c1, c2, f = np.array([0.8, 0.3, 0.6]), np.array([0.2, 0.9, 0.4]), 0.35
+    want = {"MIX": c1 * (1 - f) + c2 * f, "ADD": c1 + f * c2, "MULTIPLY": c1 * (1 - f + f * c2),
+            "SCREEN": 1 - (1 - f * c1) * (1 - c1),  # the reference's SCREEN takes color1 twice
+            "OVERLAY": np.where(c1 < 0.5, c1 * (1 - f + 2 * f * c2), 1 - (1 - f + 2 * f * (1 - c2)) * (1 - c1)),
+            "LIGHTEN": c1 * (1 - f) + np.maximum(c1, c2) * f, "DARKEN": c1 * (1 - f) + np.minimum(c1, c2) * f}[blend]
+    for clamp in (False, True):
+        img = render_value({"N": node("MIX_RGB", {"Fac": _in("VALUE", f), "Color1": _in("RGBA", list(c1) + [1.0]),
+                                                  "Color2": _in("RGBA", list(c2) + [1.0])},
+                                      {"blend_type": blend, "use_clamp": clamp})}, ("N", "Color"))
+        check(img, np.clip(want, 0, 1) if clamp else want, rel=1e-5)
+
+
+def test_mix_shader_mixes_emission():
+    """MixShader mixes the two shaders' emissions analytically
+    (nodes.hpp:1916-1931): (1 - fac) A + fac B in every sample."""
+    graph = {
+        "A": node("EMISSION", {"Color": _in("RGBA", [1.0, 0.5, 0.25, 1]), "Strength": _in("VALUE", 2.0),
+                               "Weight": _in("VALUE", 0.0)}),
+        "B": node("EMISSION", {"Color": _in("RGBA", [0.1, 0.7, 0.9, 1]), "Strength": _in("VALUE", 1.0),
+                               "Weight": _in("VALUE", 0.0)}),
+        "M": node("MIX_SHADER", {"Fac": _in("VALUE", 0.3), "Shader": _in("SHADER", link=("A", "Emission")),
+                                 "Shader_001": _in("SHADER", link=("B", "Emission"))}),
+        "Material Output": scenes.output_material(("M", "Shader")),
+    }
+    d = emitter_scene({}, ("N", "Color"))
+    d.surface_materials = [{"name": "mix", "nodes": graph}]
+    orc = binding.OracleScene(d)
+    orc.build_envmap()
+    img, _ = orc.render(4, 4, 1, frames=3)
+    want = 0.7 * np.array([2.0, 1.0, 0.5]) + 0.3 * np.array([0.1, 0.7, 0.9])
+    check(img[..., :3] / 3.0, want, rel=1e-5)
+
+
+def test_noise_4d_through_codegen():
+    """noise_dimensions '4D' evaluates fBm of (vector, w) x scale on the 4-D
+    Perlin noise (nodes.hpp:909-911), pinned separately in test_pins_cpu."""
+    from tests import devlib
+    vec, w, scale, detail, rough = [0.3, 0.7, 1.1], 0.4, 2.5, 2.0, 0.6
+    # an unlinked Vector means Generated coordinates: feed a constant through CombineXYZ
+    const = node("COMBXYZ", {"X": _in("VALUE", vec[0]), "Y": _in("VALUE", vec[1]), "Z": _in("VALUE", vec[2])})
+
+    def noise(dim):
+        return render_value({"V": const, "N": node("TEX_NOISE", {
+            "Vector": _in("VECTOR", [0, 0, 0], link=("V", "Vector")), "W": _in("VALUE", w),
+            "Scale": _in("VALUE", scale), "Detail": _in("VALUE", detail), "Roughness": _in("VALUE", rough),
+            "Distortion": _in("VALUE", 0.0)}, {"noise_dimensions": dim})}, ("N", "Fac"))
+
+    img = noise("4D")
+    p = np.array([vec[0] * scale, vec[1] * scale, vec[2] * scale, w * scale], np.float32)
+    want = devlib.fractal4(p, detail, rough)
+    check(img, [want] * 3, rel=1e-6)
+    img3 = noise("3D")
+    assert abs(img3[0, 0, 0] - want) > 1e-4  # 4-D is not the 3-D texture

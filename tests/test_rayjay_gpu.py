@@ -11,6 +11,7 @@ import raymond_amd.api as api
 import raymond_amd.rayjay as rayjay
 import raymond_amd.scenes as scenes
 from oracle import binding
+from tests import parity
 
 from .test_rayjay_cpu import lamp_scene_json
 
@@ -29,17 +30,18 @@ def loaded(device, tmp_path_factory):
 
 def per_sample(scene, orc, W, H, depth, frames):
     r = api.Renderer(scene, W, H, max_depth=depth)
-    prev = np.zeros((H, W, 4), np.float32)
+    r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
+    total = np.zeros((H, W, 4), np.float32)
     for frame in range(frames):
         st = r.execute(1)
         img = r.image()
-        sample = img - prev
-        prev = img
+        sample = img
+        total += img
         ref, res = orc.render(W, H, depth, frames=1, frame_begin=frame)
         assert [st.rays[d] for d in range(depth)] == [res.rays[d] for d in range(depth)]
         assert [st.shadow_rays[d] for d in range(depth)] == [res.shadow_rays[d] for d in range(depth)]
-        assert np.abs(sample - ref).max() <= 1e-4 * max(np.abs(ref).max(), 1e-6) + 1e-6
-    return prev
+        parity.assert_radiance_close(sample, ref)
+    return total
 
 
 def test_loaded_scene_matches_the_oracle_per_sample(loaded):
