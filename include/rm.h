@@ -139,6 +139,17 @@ typedef struct rm_camera_desc {
     float shift[2];
 } rm_camera_desc;
 
+/* An image of the material library (the reference's texture slots,
+ * MaterialBuilder.swift:889-903): channels 1 (grey) or 4 (RGBA) floats per
+ * texel, width x height texels, row 0 first (texture coordinate v from 0 at
+ * row 0).  Slot i of the generated material code reads textures[i]; the
+ * slots of a material library are listed by rmh_codegen_textures. */
+typedef struct rm_texture_desc {
+    uint32_t width, height;
+    uint32_t channels;
+    const float* pixels;
+} rm_texture_desc;
+
 typedef struct rm_scene_desc {
     uint32_t num_vertices;
     const float* vertices;      /* 3 per vertex, object space */
@@ -160,6 +171,8 @@ typedef struct rm_scene_desc {
     uint32_t world_shader;      /* light-material index of the world (LightBuilder.swift:247-248) */
     uint32_t envmap_resolution; /* 0 = 2048 (LightBuilder.swift:90) */
     rm_camera_desc camera;
+    uint32_t num_textures;
+    const rm_texture_desc* textures;
 } rm_scene_desc;
 
 /* Copies the description, uploads geometry and light tables

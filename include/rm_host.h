@@ -24,6 +24,15 @@ extern "C" {
  * (cycle, multi-input link, unsupported node; MaterialBuilder.swift:45-51).
  * noinline != 0 emits material functions as out-of-line calls. */
 int rmh_codegen(const char* materials_json, int noinline, char** out_source, char** out_error);
+/* The texture slots the generated code reads, as a malloc'd JSON array in
+ * slot order (the reference's registerTexture, MaterialBuilder.swift:
+ * 329-435): {"slot", "kind": "image" with "path" (the exporter's file path),
+ * or "nishita" with "sun_elevation", "altitude", "air_density",
+ * "dust_density", "ozone_density" of a 512 x 512 sky texture to fill with
+ * rmh_sky_nishita_texture}.  The caller passes the loaded or generated
+ * images to rm_scene_create in that order.  Returns 0, or -1 and a malloc'd
+ * error as rmh_codegen. */
+int rmh_codegen_textures(const char* materials_json, char** out_json, char** out_error);
 /* Identifier the codegen derives from a node, socket or material name
  * ("Base Color" -> "baseColor", MaterialBuilder.swift:19-43). */
 int rmh_identifier(const char* name, char* out, size_t capacity);
@@ -80,6 +89,18 @@ void rmh_ply_free(rmh_ply_mesh* mesh);
  * with rmh_free). */
 int rmh_save_exr(const char* path, const float* rgba, uint32_t width, uint32_t height, float norm,
                  char** out_error);
+
+/* Nishita sky (replaces host/sky/sky_nishita.cpp as SkyLoader.mm:8-39 and
+ * MaterialBuilder.swift:329-378 use it).  rmh_sky_nishita_texture fills a
+ * width x height RGBA float texture (alpha untouched) with the CIE XYZ sky
+ * radiance, texel (x, y) at azimuth 2 pi x / width - pi and elevation
+ * (pi/2 + pi/(4 height)) (y / height)^2; rmh_sky_nishita_sun gives the XYZ
+ * radiance of the sun disc's lower and upper edge.  Angles in radians,
+ * altitude in metres.  Return 0, or -1 on a null or empty argument. */
+int rmh_sky_nishita_texture(float* pixels, int width, int height, float sun_elevation, float altitude,
+                            float air_density, float dust_density, float ozone_density);
+int rmh_sky_nishita_sun(float sun_elevation, float angular_diameter, float altitude, float air_density,
+                        float dust_density, float* bottom, float* top);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,8 @@ struct or_scene {
     std::vector<float> light_faces;
     std::vector<float> env_pdfs, env_mip;
     int env_res = 2048;
+    std::vector<std::vector<float>> tex_pixels;  // texture slots of the material library
+    std::vector<rm::Texture> tex;
     rm::SceneView view{};
     std::vector<WTri> tris;
     std::vector<BNode> bvh;
@@ -656,6 +658,18 @@ or_scene* or_scene_create(const rm_scene_desc* d, const char* material_so, char*
     v.texcoords = s->texcoords.data();
     v.instances = s->instances.data();
     v.materials = s->materials.data();
+    for (uint32_t i = 0; i < d->num_textures; i++) {
+        const rm_texture_desc& t = d->textures[i];
+        if (!t.pixels || t.width == 0 || t.height == 0 || (t.channels != 1 && t.channels != 4)) {
+            if (s->so) dlclose(s->so);
+            delete s;
+            return fail("bad texture " + std::to_string(i));
+        }
+        s->tex_pixels.emplace_back(t.pixels, t.pixels + (size_t)t.width * t.height * t.channels);
+        s->tex.push_back(rm::Texture{s->tex_pixels.back().data(), (int)t.width, (int)t.height, (int)t.channels});
+    }
+    v.textures = s->tex.data();
+    v.num_textures = d->num_textures;
     v.camera.transform = mat4_of(d->camera.transform);
     v.camera.near_clip = d->camera.near_clip;
     v.camera.far_clip = d->camera.far_clip;

@@ -63,6 +63,11 @@ class CameraDesc(C.Structure):
                 ("focal_length", C.c_float), ("shift", C.c_float * 2)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("pixels", C.POINTER(C.c_float))]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("num_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
@@ -75,6 +80,7 @@ class SceneDesc(C.Structure):
         ("num_spot_lights", C.c_uint32), ("spot_lights", C.POINTER(SpotLight)),
         ("world_shader", C.c_uint32), ("envmap_resolution", C.c_uint32),
         ("camera", CameraDesc),
+        ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
     ]
 
 
@@ -191,6 +197,10 @@ EXPORTS = {
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
     "rmh_codegen": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rmh_identifier": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+    "rmh_codegen_textures": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "rmh_sky_nishita_texture": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          C.c_float]),
+    "rmh_sky_nishita_sun": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p]),
     "rmh_free": (None, [C.c_void_p]),
     "rmh_ply_read": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "rmh_ply_free": (None, [C.c_void_p]),
@@ -236,6 +246,42 @@ def codegen(materials_json, noinline=False):
     out = C.cast(src, C.c_char_p).value.decode()
     LIB.rmh_free(src)
     return out
+
+
+def codegen_textures(materials_json):
+    """The texture slots of a material library, in slot order
+    (rmh_codegen_textures): dicts with "kind" "image" and "path", or "nishita"
+    and the sky's atmosphere."""
+    import json
+    out = C.c_void_p()
+    err = C.c_void_p()
+    if LIB.rmh_codegen_textures(materials_json.encode(), C.byref(out), C.byref(err)) != 0:
+        msg = C.cast(err, C.c_char_p).value.decode() if err.value else "codegen failed"
+        LIB.rmh_free(err)
+        raise RmError(msg)
+    text = C.cast(out, C.c_char_p).value.decode()
+    LIB.rmh_free(out)
+    return json.loads(text)
+
+
+def sky_nishita_texture(width=512, height=512, sun_elevation=0.2617994, altitude=0.0, air_density=1.0,
+                        dust_density=1.0, ozone_density=1.0):
+    """The Nishita sky texture (rmh_sky_nishita_texture): (height, width, 4)
+    float32, CIE XYZ in RGB, alpha 1."""
+    out = np.zeros((height, width, 4), np.float32)
+    out[..., 3] = 1.0
+    _check(LIB.rmh_sky_nishita_texture(out.ctypes.data, width, height, sun_elevation, altitude, air_density,
+                                       dust_density, ozone_density))
+    return out
+
+
+def sky_nishita_sun(sun_elevation, angular_diameter, altitude=0.0, air_density=1.0, dust_density=1.0):
+    """XYZ radiance of the sun disc's lower and upper edge (rmh_sky_nishita_sun)."""
+    bottom = np.zeros(3, np.float32)
+    top = np.zeros(3, np.float32)
+    _check(LIB.rmh_sky_nishita_sun(sun_elevation, angular_diameter, altitude, air_density, dust_density,
+                                   bottom.ctypes.data, top.ctypes.data))
+    return bottom, top
 
 
 class PlyMesh(C.Structure):

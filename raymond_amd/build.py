@@ -29,7 +29,7 @@ HOST_FLAGS = [
 
 KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip", "kernels/rcache.hip"]
 HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/rcache.cpp", "host/batch.cpp",
-        "host/capi.cpp", "host/ply_reader.cpp", "host/exr_writer.cpp"]
+        "host/capi.cpp", "host/ply_reader.cpp", "host/exr_writer.cpp", "host/sky.cpp"]
 
 
 def _run(cmd):

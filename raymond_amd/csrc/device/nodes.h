@@ -8,6 +8,8 @@
 #include "elem.h"
 #include "noise.h"
 #include "shading.h"
+#include "texture.h"
+#include "warp.h"
 
 namespace rm {
 
@@ -571,6 +573,324 @@ RM_HD int floor_to_int(float x) {
     return (int)f;
 }
 
+// ---------------------------------------------------------------------------
+// Image textures and the Nishita sky (device/nodes/nodes.hpp:443-564,
+// 646-870): lookups in the scene's texture slots.
+
+// sRGB transfer function to linear (IEC 61966-2-1)
+RM_HD float srgb_to_linear(float c) {
+    if (c < 0.04045f) return c < 0.0f ? 0.0f : c * (1.0f / 12.92f);
+    return pow_((c + 0.055f) * (1.0f / 1.055f), 2.4f);
+}
+// CIE XYZ to linear sRGB primaries, D65 (IEC 61966-2-1)
+RM_HD vec3 xyz_to_rgb(vec3 v) {
+    return vec3((3.240479f * v.x + -1.537150f * v.y) + -0.498535f * v.z,
+                (-0.969256f * v.x + 1.875991f * v.y) + 0.041556f * v.z,
+                (0.055648f * v.x + -0.204043f * v.y) + 1.057311f * v.z);
+}
+
+struct kTexImage {
+    enum Interpolation { INTERPOLATION_LINEAR };
+    enum Projection { PROJECTION_FLAT, PROJECTION_BOX, PROJECTION_EQUIRECTANGULAR, PROJECTION_MIRROR_BALL };
+    enum Extension { EXTENSION_REPEAT };
+    enum Alpha { ALPHA_STRAIGHT };
+    enum ColorSpace {
+        COLOR_SPACE_LINEAR, COLOR_SPACE_SRGB, COLOR_SPACE_NON_COLOR, COLOR_SPACE_RAW, COLOR_SPACE_XYZ,
+        COLOR_SPACE_FILMIC_LOG
+    };
+};
+
+// Image and environment textures: flat projection flips v, equirectangular
+// maps a direction to (longitude, polar angle); box and mirror-ball
+// projections are not implemented by the reference (its coordinate is left
+// unset) and sample (0, 0) here.  The texel format (grey or RGBA) is the
+// slot's, read at run time.
+template <int Slot, int Interpolation, int Projection, int Extension, int Alpha, int ColorSpace>
+struct TexImage {
+    vec3 vector;
+    vec4 color;
+    float alpha = 0.0f;
+    RM_HD void compute(const SceneView& ctx, ShadingContext&) {
+        vec2 projected(0.0f, 0.0f);
+        if (Projection == kTexImage::PROJECTION_FLAT) projected = vec2(vector.x, 1.0f - vector.y);
+        if (Projection == kTexImage::PROJECTION_EQUIRECTANGULAR) projected = warp::equirect_sphere_to_square(vector);
+        color = texture_sample(ctx.textures[Slot], projected);
+        alpha = color.w;
+        if (ColorSpace == kTexImage::COLOR_SPACE_SRGB)
+            color = vec4(srgb_to_linear(color.x), srgb_to_linear(color.y), srgb_to_linear(color.z), color.w);
+        if (ColorSpace == kTexImage::COLOR_SPACE_XYZ) color = vec4(xyz_to_rgb(color.xyz()), color.w);
+    }
+};
+
+// Nishita sky radiance from the precomputed texture (longitude x sqrt of
+// elevation, CIE XYZ) and the sun's two edge colours: data = bottom XYZ (3),
+// top XYZ (3), sun elevation, sun rotation, angular diameter (< 0: no disc),
+// sun intensity (Blender's Cycles sky kernel, node_sky_texture.osl)
+RM_HD vec3 geographical_to_direction(float lat, float lon) {
+    const float cl = cos_(lat);
+    return vec3(cl * cos_(lon), cl * sin_(lon), sin_(lat));
+}
+RM_HD vec3 sky_radiance_nishita(vec3 dir, const float* data, const Texture& tex) {
+    const float sun_elevation = data[6], sun_rotation = data[7], angular_diameter = data[8], sun_intensity = data[9];
+    const bool sun_disc = angular_diameter > 0.0f;
+    const float polar = acos_(dir.z), azimuth = atan2_(dir.x, dir.y);
+    vec3 xyz(0.0f);
+    if (dir.z >= 0.0f) {
+        const vec3 sun_dir = geographical_to_direction(sun_elevation, sun_rotation + 0.5f * kPi);
+        // the angle between the two unit vectors, accurate near 0
+        const float sun_angle = 2.0f * atan2_(length(dir - sun_dir), length(dir + sun_dir));
+        const float half_angular = angular_diameter / 2.0f;
+        const float dir_elevation = 0.5f * kPi - polar;
+        if (sun_angle < half_angular && sun_disc) {
+            const vec3 bottom(data[0], data[1], data[2]), top(data[3], data[4], data[5]);
+            if (sun_elevation - half_angular > 0.0f) {
+                if (sun_elevation + half_angular > 0.0f) {
+                    const float y = (dir_elevation - sun_elevation) / angular_diameter + 0.5f;
+                    xyz = lerp(bottom, top, y) * sun_intensity;
+                }
+            } else if (sun_elevation + half_angular > 0.0f) {
+                const float y = dir_elevation / (sun_elevation + half_angular);
+                xyz = lerp(bottom, top, y) * sun_intensity;
+            }
+            // limb darkening with coefficient 0.6
+            const float f = sun_angle / half_angular;
+            xyz *= 1.0f - 0.6f * (1.0f - sqrt_(1.0f - f * f));
+        } else {
+            float x = (azimuth + kPi + sun_rotation) / (2.0f * kPi);
+            const float y = sqrt_(dir_elevation / (0.5f * kPi));  // more texels toward the horizon
+            if (x > 1.0f) x = x - 1.0f;
+            xyz = texture_sample(tex, vec2(x, y)).xyz();
+        }
+    } else if (dir.z >= -0.4f) {
+        // the ground fades to black below the horizon
+        const float fade = pow_(1.0f + dir.z * 2.5f, 3.0f);
+        float x = (azimuth + kPi + sun_rotation) / (2.0f * kPi);
+        if (x > 1.0f) x = x - 1.0f;
+        xyz = texture_sample(tex, vec2(x, 1e-3f)).xyz() * fade;
+    }
+    return xyz_to_rgb(xyz);
+}
+
+// (device/nodes/nodes.hpp:857-870); the direction is the outgoing one in the
+// sky texture's frame (y and z flipped)
+template <int Slot>
+struct TexNishita {
+    float scale = 0.0f;
+    vec3 vector;
+    vec4 color;
+    float data[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    RM_HD void compute(const SceneView& ctx, ShadingContext& s) {
+        const vec3 d = s.wo * vec3(1.0f, -1.0f, -1.0f);
+        color = vec4(scale * sky_radiance_nishita(d, data, ctx.textures[Slot]), 1.0f);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Nodes the reference maps to constant stubs (device/nodes/nodes.hpp:48-150,
+// 407-441, 566-644, 1051-1110, 1736-1744): they compile and produce the
+// reference's fixed outputs, so scenes that use them render as the
+// reference renders them.
+
+struct ObjectInfo {
+    float random = 0.0f;
+    vec3 location;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+struct AmbientOcclusion {
+    vec4 color;
+    float distance = 0.0f;
+    vec3 normal;
+    float ao = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) { ao = 1.0f; }
+};
+
+// volume closures are not rendered (nodes.hpp:71-119)
+struct VolumeScatter {
+    float anisotropy = 0.0f, density = 0.0f, weight = 0.0f;
+    vec4 color;
+    UberShader volume;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+struct VolumeAbsorption {
+    float density = 0.0f, weight = 0.0f;
+    vec4 color;
+    UberShader volume;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+struct PrincipledVolume {
+    vec4 absorptionColor, blackbodyTint, color, emissionColor;
+    float anisotropy = 0.0f, blackbodyIntensity = 0.0f, density = 0.0f, emissionStrength = 0.0f;
+    float temperature = 0.0f, weight = 0.0f;
+    UberShader volume;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+struct ParticleInfo {
+    float random = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+struct LightFalloff {
+    float strength = 0.0f, smooth = 0.0f, quadratic = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) { quadratic = 1.0f; }
+};
+
+struct VertexColor {
+    vec3 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+struct UVMapCoordinate {
+    vec3 uv;
+    RM_HD void compute(const SceneView&, ShadingContext& s) { uv = s.uv; }
+};
+
+// the curve mapping is not applied: colour and factor pass through
+struct ColorCurves {
+    vec4 color;
+    float fac = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) {}
+};
+
+// the "Normal" node's dot output is fixed at 1
+struct NormalProduct {
+    vec3 normal;
+    float dot = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) { dot = 1.0f; }
+};
+
+struct kTexWave {
+    enum Type { TYPE_BANDS };
+    enum Direction { DIRECTION_DIAGONAL, DIRECTION_X };
+    enum Profile { PROFILE_SIN };
+};
+template <int Type, int Direction, int Profile>
+struct TexWave {
+    float detail = 0.0f, detailRoughness = 0.0f, detailScale = 0.0f, distortion = 0.0f, phaseOffset = 0.0f;
+    float scale = 0.0f;
+    vec3 vector;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) { color = vec4(1.0f); }
+};
+
+struct TexIES {
+    vec3 vector;
+    float strength = 0.0f, fac = 0.0f;
+    RM_HD void compute(const SceneView&, ShadingContext&) { fac = strength; }
+};
+
+struct TexMagic {
+    float distortion = 0.0f, scale = 0.0f;
+    vec3 vector;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) { color = vec4(1.0f); }
+};
+
+struct TexMusgrave {
+    float detail = 0.0f, dimension = 0.0f, gain = 0.0f, lacunarity = 0.0f, offset = 0.0f, scale = 0.0f, w = 0.0f;
+    vec3 vector;
+    float fac = 0.0f;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        fac = 1.0f;
+        color = vec4(1.0f);
+    }
+};
+
+struct TexBrick {
+    vec4 color1, color2, mortar;
+    float bias = 0.0f, brickWidth = 0.0f, mortarSize = 0.0f, mortarSmooth = 0.0f, rowHeight = 0.0f, scale = 0.0f;
+    vec3 vector;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext&) { color = vec4(1.0f); }
+};
+
+struct Displacement {
+    float height = 0.0f, midlevel = 0.0f, scale = 0.0f;
+    vec3 normal, displacement;
+    RM_HD void compute(const SceneView&, ShadingContext&) { displacement = vec3(0.0f); }
+};
+
+// "Attribute" returns the generated coordinates and white (nodes.hpp:1736-1744)
+struct Attribute {
+    vec3 vector;
+    vec4 color;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        vector = s.generated;
+        color = vec4(1.0f);
+    }
+};
+
+// Tangent-space (or world-space) normal map (nodes.hpp:1060-1097): the
+// colour maps to [-1, 1]^3, is blended towards +z by strength and taken
+// through the frame (tu, tv, normal).
+struct kNormalMap { enum Space { SPACE_TANGENT, SPACE_WORLD }; };
+template <int Space>
+struct NormalMap {
+    vec4 color;
+    float strength = 0.0f;
+    vec3 normal;
+    RM_HD void compute(const SceneView&, ShadingContext& s) {
+        const float k = max_(strength, 0.0f);
+        normal = normalize(2.0f * color.xyz() - 1.0f);
+        normal = k * normal + (1.0f - k) * vec3(0.0f, 0.0f, 1.0f);
+        normal = normalize(normal);
+        if (Space == kNormalMap::SPACE_TANGENT) {
+            mat3 onb;
+            onb.c0 = s.tu;
+            onb.c1 = s.tv;
+            onb.c2 = s.normal;
+            normal = onb * normal;
+        }
+    }
+};
+
+// Cycles' piecewise fit of the blackbody colour over 800-12000 K, normalised
+// to unit luminance (nodes.hpp:1421-1495; Cycles' svm math_util.h): a/t +
+// b t + c for red and green, a cubic in t for blue.
+struct Blackbody {
+    float temperature = 0.0f;
+    vec4 color;
+    RM_HD static vec3 rgb(float t) {
+        if (t >= 12000.0f) return vec3(0.8262954810464208f, 0.9945080501520986f, 1.566307710274283f);
+        if (t < 800.0f) return vec3(5.413294490189271f, -0.20319390035873933f, -0.0822535242887164f);
+        const int i = (t >= 6365.0f) ? 6 : (t >= 3315.0f) ? 5 : (t >= 1902.0f) ? 4 : (t >= 1449.0f) ? 3
+                    : (t >= 1167.0f) ? 2 : (t >= 965.0f) ? 1 : 0;
+        // row i of the fit: red and green (a, b, c), blue (a, b, c, d); the
+        // rows are picked by value so that no table is indexed at run time
+        auto P = [i](float c0, float c1, float c2, float c3, float c4, float c5, float c6) {
+            return i == 0 ? c0 : i == 1 ? c1 : i == 2 ? c2 : i == 3 ? c3 : i == 4 ? c4 : i == 5 ? c5 : c6;
+        };
+        const float ra = P(1.61919106e+03f, 2.48845471e+03f, 3.34143193e+03f, 4.09461742e+03f, 4.67028036e+03f,
+                           4.59509185e+03f, 3.78717450e+03f);
+        const float rb = P(-2.05010916e-03f, -1.11330907e-03f, -4.86551192e-04f, -1.27446582e-04f, 2.91258199e-05f,
+                           2.87495649e-05f, 9.35907826e-06f);
+        const float rc = P(5.02995757e+00f, 3.22621544e+00f, 1.76486769e+00f, 7.25731635e-01f, 1.26703442e-01f,
+                           1.50345020e-01f, 3.99075871e-01f);
+        const float ga = P(-4.88999748e+02f, -7.55994277e+02f, -1.02363977e+03f, -1.26571316e+03f, -1.42529332e+03f,
+                           -1.17554822e+03f, -5.00799571e+02f);
+        const float gb = P(6.04330754e-04f, 3.16730098e-04f, 1.20223470e-04f, 4.87340896e-06f, -4.01150431e-05f,
+                           -2.16378048e-05f, -4.59832026e-06f);
+        const float gc = P(-7.55807526e-02f, 4.78306139e-01f, 9.36662319e-01f, 1.27054498e+00f, 1.43972784e+00f,
+                           1.30408023e+00f, 1.09098763e+00f);
+        const float ba = P(5.96945309e-11f, 2.40430366e-11f, -1.40949732e-11f, -3.61460868e-11f, -1.97075738e-11f,
+                           -1.61997957e-13f, 6.72650283e-13f);
+        const float bb = P(-4.85742887e-08f, 5.55021075e-08f, 1.89878968e-07f, 2.84822009e-07f, 1.75359352e-07f,
+                           -1.64216008e-08f, -2.73078809e-08f);
+        const float bc = P(-9.70622247e-05f, -1.98503712e-04f, -3.56632824e-04f, -4.93211319e-04f, -2.50542825e-04f,
+                           3.86216271e-04f, 4.24098264e-04f);
+        const float bd = P(-4.07936148e-03f, 2.89312858e-02f, 9.10767778e-02f, 1.56723440e-01f, -2.22783266e-02f,
+                           -7.38077418e-01f, -7.52335691e-01f);
+        const float inv = 1.0f / t;
+        return vec3((ra * inv + rb * t) + rc, (ga * inv + gb * t) + gc, ((ba * t + bb) * t + bc) * t + bd);
+    }
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        const vec3 b = rgb(temperature);
+        color = vec4(b / luminance(b), 1.0f);
+    }
+};
+
 // (device/nodes/nodes.hpp:247-264)
 struct TexChecker {
     float scale = 0.0f;
@@ -876,6 +1196,22 @@ struct BsdfTransparent {
     RM_HD void compute(const SceneView&, ShadingContext&) {
         bsdf.alpha = 0.0f;
         bsdf.alpha_weight = color.xyz();
+    }
+};
+
+// Hair is shaded as a diffuse lobe of its colour about the tangent
+// (nodes.hpp:1849-1869)
+struct BsdfHair {
+    vec4 color;
+    float offset = 0.0f, roughnessu = 0.0f, roughnessv = 0.0f, weight = 0.0f;
+    vec3 tangent;
+    UberShader bsdf;
+    RM_HD void compute(const SceneView&, ShadingContext&) {
+        bsdf.diffuse.diffuse_weight = color.xyz();
+        bsdf.diffuse.sheen_weight = vec3(0.0f);
+        bsdf.diffuse.roughness = 0.0f;
+        bsdf.normal = tangent;
+        bsdf.lobe_prob[0] = 1.0f;
     }
 };
 

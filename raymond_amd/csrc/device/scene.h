@@ -91,6 +91,13 @@ struct LightsView {
     const ShapeLight* shape;
 };
 
+// An image of the material library (the reference's ctx.textures[i]):
+// width x height texels of `channels` floats (1: R, 4: RGBA), row 0 first.
+struct Texture {
+    const float* pixels;
+    int width, height, channels;
+};
+
 struct SceneView {
     const float* vertices;     // packed float3, object space
     const uint32_t* indices;   // 3 per face, shape-local vertex indices
@@ -100,6 +107,8 @@ struct SceneView {
     const uint16_t* materials; // per face
     Camera camera;
     LightsView lights;
+    const Texture* textures;   // slots of the material library's images
+    uint32_t num_textures;
 
     RM_HD vec3 vertex(uint32_t i) const { return vec3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]); }
     RM_HD vec3 normal(uint32_t i) const { return vec3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]); }

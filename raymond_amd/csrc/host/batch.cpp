@@ -153,6 +153,9 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
     v.instances = m->d_instances.as<PerInstance>();
     v.materials = m->d_materials.as<uint16_t>();
     v.camera = jobs[0].scene->view.camera;
+    // the jobs share one material library, whose texture slots job 0's scene holds
+    v.textures = jobs[0].scene->view.textures;
+    v.num_textures = jobs[0].scene->view.num_textures;
     std::vector<LightsView> lights(n);
     for (uint32_t j = 0; j < n; j++) {
         lights[j] = jobs[j].scene->view.lights;

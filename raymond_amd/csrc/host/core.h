@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../../include/rm.h"
@@ -123,6 +124,8 @@ struct rm_scene {
     rm::DevBuf d_env_pdfs, d_env_mip;
     rm::DevBuf d_spill;
     size_t spill_threads = 0;
+    std::vector<std::unique_ptr<rm::DevBuf>> d_tex;  // texel arrays of the texture slots
+    rm::DevBuf d_tex_table;                          // rm::Texture per slot
     rm::SceneView view{};
     bool accel_built = false;
     bool materials_built = false;

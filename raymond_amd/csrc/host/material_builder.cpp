@@ -22,6 +22,10 @@
 #include "json.h"
 
 namespace rm {
+namespace sky {
+void precompute_sun(float sun_elevation, float angular_diameter, float altitude, float air_density,
+                    float dust_density, float* bottom, float* top);
+}
 namespace {
 
 using json::Value;
@@ -101,6 +105,12 @@ std::string param_str(const Value& node, const char* key, const char* def) {
     const Value* v = param(node, key);
     return v && v->is_string() ? v->str : def;
 }
+double param_num(const Value& node, const char* key, double def) {
+    const Value* v = param(node, key);
+    if (v && v->is_number()) return v->num;
+    if (v && v->kind == Value::Bool) return v->b ? 1.0 : 0.0;
+    return def;
+}
 bool param_bool(const Value& node, const char* key) {
     const Value* v = param(node, key);
     return v && ((v->kind == Value::Bool && v->b) || (v->is_number() && v->num != 0));
@@ -113,8 +123,37 @@ struct Invocation {
     std::string comment;
 };
 
+// A texture slot of the material library: an image file, or the Nishita
+// sky texture of the given atmosphere (512 x 512, MaterialBuilder.swift:351-372).
+struct TextureSlot {
+    std::string kind;  // "image" or "nishita"
+    std::string path;
+    float sun_elevation = 0.0f, altitude = 0.0f, air_density = 0.0f, dust_density = 0.0f, ozone_density = 0.0f;
+    bool same(const TextureSlot& o) const {
+        return kind == o.kind && path == o.path && sun_elevation == o.sun_elevation && altitude == o.altitude &&
+               air_density == o.air_density && dust_density == o.dust_density && ozone_density == o.ozone_density;
+    }
+};
+
+std::string json_string(const std::string& s) {
+    std::string out = "\"";
+    for (char c : s) {
+        if (c == '"' || c == '\\') out += '\\';
+        if ((unsigned char)c < 0x20) {
+            char buf[8];
+            snprintf(buf, sizeof(buf), "\\u%04x", (unsigned)(unsigned char)c);
+            out += buf;
+            continue;
+        }
+        out += c;
+    }
+    return out + "\"";
+}
+
 class Codegen {
   public:
+    const std::vector<TextureSlot>& textures() const { return textures_; }
+
     void add_table(const std::string& fn, const Value& materials) {
         if (!materials.is_array()) throw CodegenError("material table must be an array");
         std::vector<std::string> fns;
@@ -148,6 +187,7 @@ class Codegen {
     }
 
   private:
+    std::vector<TextureSlot> textures_;  // slots in first-use order
     std::vector<std::pair<std::string, std::string>> functions_;  // name, definition
     std::vector<std::pair<std::string, std::vector<std::string>>> tables_;
     std::set<std::string> used_names_;
@@ -190,6 +230,100 @@ class Codegen {
         text += "}\n\n";
         functions_.push_back({fn, text});
         return fn;
+    }
+
+    // Node types with a kernel of fixed form (MaterialBuilder.swift:442-603);
+    // most are the reference's constant stubs.
+    static const char* stub_kernel(const std::string& t) {
+        static const std::map<std::string, const char*> kernels = {
+            {"TEX_BRICK", "TexBrick"}, {"TEX_MAGIC", "TexMagic"}, {"TEX_MUSGRAVE", "TexMusgrave"},
+            {"TEX_IES", "TexIES"}, {"AMBIENT_OCCLUSION", "AmbientOcclusion"}, {"VOLUME_SCATTER", "VolumeScatter"},
+            {"VOLUME_ABSORPTION", "VolumeAbsorption"}, {"PRINCIPLED_VOLUME", "PrincipledVolume"},
+            {"DISPLACEMENT", "Displacement"}, {"ATTRIBUTE", "Attribute"}, {"OBJECT_INFO", "ObjectInfo"},
+            {"PARTICLE_INFO", "ParticleInfo"}, {"LIGHT_FALLOFF", "LightFalloff"}, {"VERTEX_COLOR", "VertexColor"},
+            {"NORMAL", "NormalProduct"}, {"CURVE_RGB", "ColorCurves"}, {"BLACKBODY", "Blackbody"},
+            {"UVMAP", "UVMapCoordinate"}};
+        auto it = kernels.find(t);
+        return it == kernels.end() ? nullptr : it->second;
+    }
+
+    int texture_slot(const TextureSlot& t) {
+        for (size_t i = 0; i < textures_.size(); i++)
+            if (textures_[i].same(t)) return (int)i;
+        textures_.push_back(t);
+        return (int)textures_.size() - 1;
+    }
+
+    // Image and environment textures (MaterialBuilder.swift:379-435): one slot
+    // per file, linear interpolation, repeat extension, straight alpha
+    Invocation image_texture(const Value& node, bool environment) {
+        TextureSlot ts;
+        ts.kind = "image";
+        ts.path = param_str(node, "filepath", "");
+        if (ts.path.empty()) throw CodegenError("image texture without a file");
+        const std::string interp = enum_token(param_str(node, "interpolation", "Linear"));
+        if (interp != "LINEAR") throw CodegenError("unsupported image interpolation '" + interp + "'");
+        const std::string proj = enum_token(param_str(node, "projection", environment ? "EQUIRECTANGULAR" : "FLAT"));
+        static const std::set<std::string> projections = {"FLAT", "BOX", "EQUIRECTANGULAR", "MIRROR_BALL"};
+        if (!projections.count(proj)) throw CodegenError("unsupported image projection '" + proj + "'");
+        const std::string cs = enum_token(param_str(node, "colorspace", "sRGB"));
+        static const std::set<std::string> spaces = {"LINEAR", "SRGB", "NON_COLOR", "RAW", "XYZ", "FILMIC_LOG"};
+        if (!spaces.count(cs)) throw CodegenError("unsupported colour space '" + cs + "'");
+        Invocation inv;
+        inv.kernel = "TexImage<" + std::to_string(texture_slot(ts)) + ", kTexImage::INTERPOLATION_" + interp +
+                     ", kTexImage::PROJECTION_" + proj + ", kTexImage::EXTENSION_REPEAT, kTexImage::ALPHA_STRAIGHT, "
+                     "kTexImage::COLOR_SPACE_" + cs + ">";
+        inv.comment = ts.path;
+        return inv;
+    }
+
+    // Sky texture (MaterialBuilder.swift:329-378): Nishita, or Hosek-Wilkie
+    // mapped onto Nishita the reference's ad-hoc way
+    Invocation sky_texture(const Value& node) {
+        std::string type = enum_token(param_str(node, "sky_type", "NISHITA"));
+        float elevation = (float)param_num(node, "sun_elevation", 0.2617994f);
+        float rotation = (float)param_num(node, "sun_rotation", 0.0);
+        bool disc = param(node, "sun_disc") ? param_bool(node, "sun_disc") : true;
+        const float size = (float)param_num(node, "sun_size", 0.009512044f);
+        const float intensity = (float)param_num(node, "sun_intensity", 1.0);
+        const float altitude = (float)param_num(node, "altitude", 0.0);
+        float air = (float)param_num(node, "air_density", 1.0);
+        float dust = (float)param_num(node, "dust_density", 1.0);
+        float ozone = (float)param_num(node, "ozone_density", 1.0);
+        float scale = 1.0f;
+        if (type == "HOSEK_WILKIE") {
+            float d[3] = {0.0f, 0.0f, 1.0f};
+            if (const Value* sd = param(node, "sun_direction"))
+                if (sd->is_array() && sd->arr.size() == 3)
+                    for (int k = 0; k < 3; k++) d[k] = (float)sd->arr[k].num;
+            rotation = std::atan2(d[0], d[1]);
+            elevation = std::max(std::asin(d[2]), 5.0f * 3.14159265358979323846f / 180.0f);
+            disc = false;
+            air = 2.0f;
+            dust = 0.5f;
+            ozone = 1.0f;
+            scale = 0.08f;
+            type = "NISHITA";
+        }
+        if (type != "NISHITA") throw CodegenError("unsupported sky type '" + type + "'");
+        TextureSlot ts;
+        ts.kind = "nishita";
+        ts.sun_elevation = elevation;
+        ts.altitude = altitude;
+        ts.air_density = air;
+        ts.dust_density = dust;
+        ts.ozone_density = ozone;
+        float data[10];
+        sky::precompute_sun(elevation, size, altitude, air, dust, data, data + 3);
+        data[6] = elevation;
+        data[7] = rotation;
+        data[8] = disc ? size : -1.0f;
+        data[9] = intensity;
+        Invocation inv;
+        inv.kernel = "TexNishita<" + std::to_string(texture_slot(ts)) + ">";
+        inv.in["scale"] = fmt_float(scale);
+        for (int k = 0; k < 10; k++) inv.extra.push_back("data[" + std::to_string(k) + "] = " + fmt_float(data[k]));
+        return inv;
     }
 
     // kernel type and fixed parameters per node type (MaterialBuilder.swift:327-662)
@@ -320,6 +454,28 @@ class Codegen {
             inv.kernel = "BsdfTranslucent";
         } else if (t == "BSDF_VELVET") {
             inv.kernel = "BsdfVelvet";
+        } else if (t == "TEX_IMAGE" || t == "TEX_ENVIRONMENT") {
+            inv = image_texture(node, t == "TEX_ENVIRONMENT");
+        } else if (t == "TEX_SKY") {
+            inv = sky_texture(node);
+        } else if (t == "BSDF_HAIR") {
+            inv.kernel = "BsdfHair";
+        } else if (t == "TEX_WAVE") {
+            // the reference's enums hold bands along the diagonal or x with a sine profile
+            const std::string wt = enum_token(param_str(node, "type", "BANDS"));
+            const std::string dir = enum_token(param_str(node, "direction", "X"));
+            const std::string prof = enum_token(param_str(node, "profile", "SIN"));
+            if (wt != "BANDS") throw CodegenError("unsupported wave type '" + wt + "'");
+            if (dir != "DIAGONAL" && dir != "X") throw CodegenError("unsupported wave direction '" + dir + "'");
+            if (prof != "SIN") throw CodegenError("unsupported wave profile '" + prof + "'");
+            inv.kernel = "TexWave<kTexWave::TYPE_" + wt + ", kTexWave::DIRECTION_" + dir + ", kTexWave::PROFILE_" +
+                         prof + ">";
+        } else if (t == "NORMAL_MAP") {
+            const std::string space = enum_token(param_str(node, "space", "TANGENT"));
+            if (space != "TANGENT" && space != "WORLD") throw CodegenError("unsupported normal map space '" + space + "'");
+            inv.kernel = "NormalMap<kNormalMap::SPACE_" + space + ">";
+        } else if (const char* stub = stub_kernel(t)) {
+            inv.kernel = stub;
         } else {
             throw CodegenError("unsupported node type '" + t + "'");
         }
@@ -404,6 +560,29 @@ std::string generate_material_source(const std::string& materials_json, bool noi
     return cg.build(noinline);
 }
 
+std::string generate_material_textures(const std::string& materials_json) {
+    const json::Value root = json::parse(materials_json);
+    Codegen cg;
+    const json::Value* surface = root.get("surface");
+    const json::Value* light = root.get("light");
+    json::Value empty;
+    empty.kind = json::Value::Array;
+    cg.add_table("shade_surface", surface ? *surface : empty);
+    cg.add_table("shade_light", light ? *light : empty);
+    std::string out = "[";
+    for (size_t i = 0; i < cg.textures().size(); i++) {
+        const TextureSlot& t = cg.textures()[i];
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "\"sun_elevation\": %.9g, \"altitude\": %.9g, \"air_density\": %.9g, \"dust_density\": %.9g, "
+                 "\"ozone_density\": %.9g",
+                 t.sun_elevation, t.altitude, t.air_density, t.dust_density, t.ozone_density);
+        out += std::string(i ? ", " : "") + "{\"slot\": " + std::to_string(i) + ", \"kind\": " + json_string(t.kind) +
+               ", \"path\": " + json_string(t.path) + ", " + buf + "}";
+    }
+    return out + "]";
+}
+
 std::string material_identifier(const std::string& name) { return make_identifier(name); }
 
 }  // namespace rm
@@ -417,6 +596,19 @@ int rmh_codegen(const char* materials_json, int noinline, char** out_source, cha
     try {
         const std::string src = rm::generate_material_source(materials_json, noinline != 0);
         *out_source = strdup(src.c_str());
+        return 0;
+    } catch (const std::exception& e) {
+        if (out_error) *out_error = strdup(e.what());
+        return -1;
+    }
+}
+
+int rmh_codegen_textures(const char* materials_json, char** out_json, char** out_error) {
+    if (out_json) *out_json = nullptr;
+    if (out_error) *out_error = nullptr;
+    if (!materials_json || !out_json) return -1;
+    try {
+        *out_json = strdup(rm::generate_material_textures(materials_json).c_str());
         return 0;
     } catch (const std::exception& e) {
         if (out_error) *out_error = strdup(e.what());

@@ -178,6 +178,19 @@ void scene_create(rm_device* dev, const rm_scene_desc* d, rm_scene* s) {
     s->d_spot.upload(s->spot.data(), s->spot.size() * sizeof(SpotLight), st);
     s->d_shape.upload(s->shape.data(), s->shape.size() * sizeof(ShapeLight), st);
     s->d_light_faces.upload(s->light_faces.data(), s->light_faces.size() * 4, st);
+    // texture slots of the material library
+    RM_REQUIRE(d->num_textures == 0 || d->textures, "textures missing");
+    std::vector<Texture> tex_table;
+    for (uint32_t i = 0; i < d->num_textures; i++) {
+        const rm_texture_desc& t = d->textures[i];
+        RM_REQUIRE(t.width > 0 && t.height > 0 && t.pixels, "texture " + std::to_string(i) + " is empty");
+        RM_REQUIRE(t.channels == 1 || t.channels == 4, "texture channels must be 1 or 4");
+        RM_REQUIRE((uint64_t)t.width * t.height * t.channels < (1ull << 31), "texture too large");
+        s->d_tex.emplace_back(new DevBuf());
+        s->d_tex.back()->upload(t.pixels, (size_t)t.width * t.height * t.channels * 4, st);
+        tex_table.push_back(Texture{s->d_tex.back()->as<float>(), (int)t.width, (int)t.height, (int)t.channels});
+    }
+    s->d_tex_table.upload(tex_table.data(), tex_table.size() * sizeof(Texture), st);
     RM_HIP(hipStreamSynchronize(st));
 
     SceneView& v = s->view;
@@ -187,6 +200,8 @@ void scene_create(rm_device* dev, const rm_scene_desc* d, rm_scene* s) {
     v.texcoords = s->d_texcoords.as<float>();
     v.instances = s->d_instances.as<PerInstance>();
     v.materials = s->d_materials.as<uint16_t>();
+    v.textures = s->d_tex_table.as<Texture>();
+    v.num_textures = d->num_textures;
     LightsView& L = v.lights;
     L.num_area = (int)s->area.size();
     L.num_point = (int)s->point.size();

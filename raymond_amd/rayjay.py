@@ -161,6 +161,8 @@ def load_scene(path, envmap_resolution=2048):
     lights = data.get("lights") or {}
     mats = _MaterialIndex(materials)
     desc = scenes.SceneDescription(os.path.splitext(os.path.basename(path))[0])
+    # image paths of the exporter are relative to the scene file
+    desc.texture_dir = os.path.dirname(os.path.abspath(path))
     desc.envmap_resolution = envmap_resolution
 
     # light materials first (LightBuilder.swift:27-34)

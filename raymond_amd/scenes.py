@@ -8,6 +8,8 @@ is not reproducible).
 """
 import ctypes as C
 import json
+import os
+import warnings
 import math
 
 import numpy as np
@@ -190,6 +192,11 @@ class SceneDescription:
         self.world_shader = 0
         self.camera = None      # dict(transform, near, far, focal, shift)
         self.envmap_resolution = 2048
+        # texture slots of the material library: None resolves them from the
+        # generated code (image files relative to texture_dir, Nishita skies
+        # generated); or a list of (h, w, 1 | 4) float arrays in slot order
+        self.textures = None
+        self.texture_dir = None
         self._keep = []
 
     def add_shape(self, shape):
@@ -208,6 +215,28 @@ class SceneDescription:
         if noinline is None:
             noinline = sum(len(m["nodes"]) for m in self.surface_materials) > 400
         return api.codegen(self.materials_json(), noinline)
+
+    def resolve_textures(self):
+        """The texture slots the generated material code reads, in slot order
+        (api.codegen_textures): image files loaded (a file that cannot be read
+        becomes one black texel, as in the reference), Nishita sky textures
+        generated at 512 x 512."""
+        from raymond_amd import images
+        out = []
+        for s in api.codegen_textures(self.materials_json()):
+            if s["kind"] == "nishita":
+                out.append(api.sky_nishita_texture(512, 512, s["sun_elevation"], s["altitude"], s["air_density"],
+                                                   s["dust_density"], s["ozone_density"]))
+                continue
+            path = s["path"]
+            if not os.path.isabs(path):
+                path = os.path.join(self.texture_dir or os.getcwd(), path)
+            try:
+                out.append(images.load(path))
+            except (OSError, ValueError) as e:
+                warnings.warn("texture %s: %s; using a black texel" % (path, e))
+                out.append(images.placeholder())
+        return out
 
     def arrays(self):
         """Concatenated geometry and per-instance ranges."""
@@ -272,6 +301,18 @@ class SceneDescription:
         sd.world_shader = self.world_shader
         sd.envmap_resolution = self.envmap_resolution
         sd.camera = self.camera_c()
+        if self.textures is None:
+            self.textures = self.resolve_textures()
+        tex = (api.TextureDesc * max(1, len(self.textures)))()
+        for k, t in enumerate(self.textures):
+            t = np.ascontiguousarray(t, np.float32)
+            if t.ndim != 3 or t.shape[2] not in (1, 4):
+                raise ValueError("texture %d must be (h, w, 1) or (h, w, 4)" % k)
+            self._keep.append(t)
+            tex[k] = api.TextureDesc(t.shape[1], t.shape[0], t.shape[2], t.ctypes.data_as(C.POINTER(C.c_float)))
+        self._keep.append(tex)
+        sd.num_textures = len(self.textures)
+        sd.textures = tex
         return sd
 
     @property

@@ -330,3 +330,21 @@ def mix_shader(fac, e0, e1, rnd, L=None):
     out = np.zeros((rnd.size, 5), np.float32)
     L.shim_mix_shader(C.c_float(fac), _p(e0), _p(e1), _p(rnd), _p(out), rnd.size)
     return out
+
+
+def texture_sample(pixels, uv, L=None):
+    """Bilinear repeat lookups of an (h, w, c) texture at uv (n, 2)."""
+    L = L or lib()
+    px = f32(pixels)
+    uv = f32(uv).reshape(-1, 2)
+    out = np.zeros((uv.shape[0], 4), np.float32)
+    L.shim_texture_sample(_p(px), px.shape[1], px.shape[0], px.shape[2], _p(uv), _p(out), uv.shape[0])
+    return out
+
+
+def sky_nishita(pixels, data, dirs, L=None):
+    L = L or lib()
+    px, data, dirs = f32(pixels), f32(data), f32(dirs).reshape(-1, 3)
+    out = np.zeros((dirs.shape[0], 3), np.float32)
+    L.shim_sky_nishita(_p(px), px.shape[1], px.shape[0], _p(data), _p(dirs), _p(out), dirs.shape[0])
+    return out

@@ -470,4 +470,23 @@ void shim_mix_shader(float fac, const float* e0, const float* e1, const float* r
     }
 }
 
+// bilinear repeat lookups of a w x h texture with `channels` floats per texel
+void shim_texture_sample(const float* pixels, int w, int h, int channels, const float* uv, float* out, int n) {
+    const Texture t{pixels, w, h, channels};
+    for (int i = 0; i < n; i++) {
+        const vec4 c = texture_sample(t, vec2(uv[2 * i], uv[2 * i + 1]));
+        out[4 * i] = c.x; out[4 * i + 1] = c.y; out[4 * i + 2] = c.z; out[4 * i + 3] = c.w;
+    }
+}
+
+// Nishita sky radiance (linear sRGB) toward directions d from the sky
+// texture and the node's ten data values
+void shim_sky_nishita(const float* pixels, int w, int h, const float* data, const float* dirs, float* out, int n) {
+    const Texture t{pixels, w, h, 4};
+    for (int i = 0; i < n; i++) {
+        const vec3 c = sky_radiance_nishita(vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), data, t);
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
 }  // extern "C"

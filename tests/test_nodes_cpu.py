@@ -302,3 +302,73 @@ def test_noise_4d_through_codegen():
     check(img, [want] * 3, rel=1e-6)
     img3 = noise("3D")
     assert abs(img3[0, 0, 0] - want) > 1e-4  # 4-D is not the 3-D texture
+
+
+# ---------------------------------------------------------------------------
+# Node kinds the reference maps to stubs or fixed forms (nodes.hpp:48-150,
+# 407-441, 566-644, 1051-1110, 1451-1495, 1736-1744, 1849-1869)
+
+@pytest.mark.parametrize("t,params,out,want", [
+    ("TEX_BRICK", {}, "Color", [1.0, 1.0, 1.0]),
+    ("TEX_MAGIC", {}, "Color", [1.0, 1.0, 1.0]),
+    ("TEX_MUSGRAVE", {"type": "FBM", "dimensions": "3D"}, "Fac", [1.0, 1.0, 1.0]),
+    ("TEX_MUSGRAVE", {}, "Color", [1.0, 1.0, 1.0]),
+    ("TEX_WAVE", {"type": "BANDS", "direction": "X", "profile": "SIN"}, "Color", [1.0, 1.0, 1.0]),
+    ("AMBIENT_OCCLUSION", {"inside": False, "only_local": False, "samples": 16}, "AO", [1.0, 1.0, 1.0]),
+    ("ATTRIBUTE", {}, "Color", [1.0, 1.0, 1.0]),
+    ("LIGHT_FALLOFF", {}, "Quadratic", [1.0, 1.0, 1.0]),
+    ("NORMAL", {}, "Dot", [1.0, 1.0, 1.0]),
+])
+def test_stub_nodes(t, params, out, want):
+    check(render_value({"N": node(t, {}, params)}, ("N", out)), want)
+
+
+def test_ies_passes_strength():
+    check(render_value({"N": node("TEX_IES", {"Strength": _in("VALUE", 0.37)})}, ("N", "Fac")), [0.37] * 3)
+
+
+def test_attribute_vector_is_generated():
+    """Attribute's vector is the generated coordinate: on the emitter quad
+    [-2, 2]^2 x {0}, (x + 2) / 4, (y + 2) / 4, 0.5 (zero extent)."""
+    img = render_value({"N": node("ATTRIBUTE", {})}, ("N", "Vector"))
+    xs = (np.arange(4) + 0.5) / 4 * 2 - 1  # pixel centres on the image plane at focal 2, distance 1
+    gx = (np.clip(xs * 0.5, -2, 2) + 2) / 4
+    assert np.allclose(img[..., 2], 0.5)
+    assert np.all(np.diff(img[0, :, 0]) > 0) and np.all(np.diff(img[:, 0, 1]) < 0)
+    assert np.abs(img[0, :, 0] - gx).max() < 0.05
+
+
+def test_wave_variants_outside_the_reference_enums_are_rejected():
+    for params in ({"type": "RINGS", "direction": "X", "profile": "SIN"},
+                   {"type": "BANDS", "direction": "Y", "profile": "SIN"},
+                   {"type": "BANDS", "direction": "X", "profile": "SAW"}):
+        nodes = {"N": node("TEX_WAVE", {}, params)}
+        d = emitter_scene(nodes, ("N", "Color"))
+        with pytest.raises(api.RmError, match="unsupported wave"):
+            d.material_source()
+
+
+def test_blackbody_unit_luminance():
+    """Blackbody colours are normalised to luminance 1; hot is bluish, cool reddish."""
+    out = {}
+    for temp in (500.0, 1000.0, 2000.0, 4000.0, 6500.0, 9000.0, 15000.0):
+        img = render_value({"N": node("BLACKBODY", {"Temperature": _in("VALUE", temp)})}, ("N", "Color"))
+        rgb = img[0, 0]
+        assert abs(0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2] - 1.0) < 1e-5, (temp, rgb)
+        out[temp] = rgb
+    assert out[2000.0][0] > out[2000.0][2] and out[15000.0][2] > out[15000.0][0]
+    assert np.abs(out[6500.0] - 1.0).max() < 0.15
+
+
+def test_normal_map_tangent_space():
+    """A tangent-space normal map of colour c and strength s gives
+    normalize(s n + (1 - s) z) in the frame (tu, tv, normal); the emitter
+    quad's frame is (x, y, z), and a flat map (0.5, 0.5, 1) returns the
+    normal itself."""
+    for color, strength in (([0.5, 0.5, 1.0, 1.0], 1.0), ([0.8, 0.3, 0.9, 1.0], 1.0), ([0.8, 0.3, 0.9, 1.0], 0.4)):
+        img = render_value({"N": node("NORMAL_MAP", {"Color": _in("RGBA", color), "Strength": _in("VALUE", strength)},
+                                      {"space": "TANGENT"})}, ("N", "Normal"))
+        n = 2 * np.array(color[:3]) - 1
+        n /= np.linalg.norm(n)
+        n = strength * n + (1 - strength) * np.array([0, 0, 1.0])
+        check(img, n / np.linalg.norm(n), rel=1e-5)

@@ -155,3 +155,77 @@ def test_node_library_matches_the_oracle_per_sample(box):
         assert [st.shadow_rays[k] for k in range(depth)] == [res.shadow_rays[k] for k in range(depth)]
         parity.assert_radiance_close(sample, ref)
     assert total[..., :3].max() > 0.0
+
+
+@pytest.fixture(scope="module")
+def textured(device):
+    """The Cornell box with an image texture (sRGB, on the walls' UVs), a
+    normal map and the stub textures on the boxes, under a Nishita sky world
+    seen through the open side; the world is importance-sampled."""
+    rng = np.random.default_rng(21)
+    img = rng.uniform(0, 1, (16, 12, 4)).astype(np.float32)
+    grey = rng.uniform(0, 1, (8, 8, 1)).astype(np.float32)
+    walls = {
+        "UV": node("TEX_COORD", {}),
+        "Img": node("TEX_IMAGE", {"Vector": _in("VECTOR", [0, 0, 0], link=("UV", "UV"))},
+                    {"filepath": "wall.png", "interpolation": "Linear", "projection": "FLAT", "colorspace": "sRGB"}),
+        "Bump": node("TEX_IMAGE", {"Vector": _in("VECTOR", [0, 0, 0], link=("UV", "UV"))},
+                     {"filepath": "grey.png", "interpolation": "Linear", "projection": "FLAT",
+                      "colorspace": "Non-Color"}),
+        "NM": node("NORMAL_MAP", {"Color": _in("RGBA", [0.5, 0.5, 1, 1], link=("Bump", "Color")),
+                                  "Strength": _in("VALUE", 0.3)}, {"space": "TANGENT"}),
+        "Diff": node("BSDF_DIFFUSE", {"Color": _in("RGBA", [0, 0, 0, 1], link=("Img", "Color")),
+                                      "Roughness": _in("VALUE", 0.2),
+                                      "Normal": _in("VECTOR", [0, 0, 0], link=("NM", "Normal"))}),
+        "Material Output": output(("Diff", "BSDF")),
+    }
+    boxes = {
+        "Brick": node("TEX_BRICK", {}), "Magic": node("TEX_MAGIC", {}),
+        "BB": node("BLACKBODY", {"Temperature": _in("VALUE", 4500.0)}),
+        "Mix": node("MIX_RGB", {"Color1": _in("RGBA", [0, 0, 0, 1], link=("BB", "Color")),
+                                "Color2": _in("RGBA", [0, 0, 0, 1], link=("Brick", "Color")),
+                                "Fac": _in("VALUE", 0.4)}, {"blend_type": "MULTIPLY"}),
+        "P": node("BSDF_PRINCIPLED", {"Base Color": _in("RGBA", [0, 0, 0, 1], link=("Mix", "Color")),
+                                      "Roughness": _in("VALUE", 0.35), "Specular": _in("VALUE", 0.5)}),
+        "Material Output": output(("P", "BSDF")),
+    }
+    sky = {
+        "S": node("TEX_SKY", {}, {"sky_type": "NISHITA", "sun_elevation": 0.35, "sun_rotation": 1.2,
+                                  "sun_disc": True, "sun_size": 0.05, "sun_intensity": 0.2, "altitude": 0.0,
+                                  "air_density": 1.0, "dust_density": 1.0, "ozone_density": 1.0}),
+        "Background": node("BACKGROUND", {"Color": _in("RGBA", [0, 0, 0, 1], link=("S", "Color")),
+                                          "Strength": _in("VALUE", 0.05)}),
+        "World Output": {"type": "OUTPUT_WORLD", "parameters": {},
+                         "inputs": {"Surface": _in("SHADER", link=("Background", "Background"))}},
+    }
+    d = scenes.cornell_box(32, 24)
+    d.surface_materials[0] = {"name": "textured walls", "nodes": walls}
+    d.surface_materials[3] = {"name": "stub boxes", "nodes": boxes}
+    d.light_materials[0] = {"name": "sky", "nodes": sky}
+    slots = api.codegen_textures(d.materials_json())
+    assert [s["kind"] for s in slots] == ["image", "image", "nishita"]
+    d.textures = [img, grey, api.sky_nishita_texture(512, 512, 0.35, 0.0, 1.0, 1.0, 1.0)]
+    d.envmap_resolution = 128
+    scene = scenes.build_device_scene(device, d)
+    orc = binding.OracleScene(d)
+    orc.build_envmap()
+    return d, scene, orc
+
+
+def test_textured_scene_matches_the_oracle_per_sample(textured):
+    d, scene, orc = textured
+    gp, gm = scene.read_envmap()
+    op, om = orc.read_envmap()
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))  # the sky's importance map
+    W, H, depth = 32, 24, 6
+    r = api.Renderer(scene, W, H, max_depth=depth)
+    r.set_uniforms(accumulate=0)
+    total = np.zeros((H, W, 4), np.float32)
+    for frame in range(3):
+        st = r.execute(1)
+        img = r.image()
+        total += img
+        ref, res = orc.render(W, H, depth, frames=1, frame_begin=frame)
+        assert [st.rays[k] for k in range(depth)] == [res.rays[k] for k in range(depth)]
+        parity.assert_radiance_close(img, ref)
+    assert total[..., :3].max() > 0.0
