@@ -7,12 +7,16 @@
 //   * triangles: the watertight test of Woop, Benthin and Wald (JCGT 2013):
 //     shear to the dominant axis, edge functions in float with a double
 //     re-evaluation when any is exactly zero, so no ray leaks between
-//     triangles sharing an edge;
+//     triangles sharing an edge.  The distance is the plane distance,
+//     within a few roundings of the exact one on planes normal to an axis
+//     (ground planes, walls);
 //   * boxes (the oracle's BVH2): slab test made conservative by scaling the
-//     exit distance by 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT
-//     2013), so a BVH can never cull a triangle that the exhaustive test
-//     would hit.  The GPU traversal tests its quantised child boxes with its
-//     own bounded-error form of the slab test and uses kBoxSlack on best_t.
+//     exit distance by 1 + 2^-20 (Ize, "Robust BVH Ray Traversal", JCGT 2013:
+//     2*gamma(3) covers the slab arithmetic; the rest covers the triangle
+//     distance's own rounding), so a BVH never culls a box holding a triangle
+//     that the exhaustive test hits.  The GPU traversal tests its quantised
+//     child boxes with its own bounded-error form of the slab test and uses
+//     kBoxSlack on best_t.
 // Closest-hit ties (equal t) go to the smaller (instance, primitive) pair, so
 // the result does not depend on traversal order.
 #pragma once
@@ -47,13 +51,17 @@ RM_HD RayShear make_shear(vec3 d) {
 RM_HD bool intersect_triangle_uvd(vec3 o, const RayShear& s, vec3 v0, vec3 v1, vec3 v2, float tmin, float tmax,
                                   float& t, float& u_out, float& v_out, float& det_out) {
     const vec3 A = v0 - o, B = v1 - o, C = v2 - o;
-    const float Az_ = comp(A, s.kz), Bz_ = comp(B, s.kz), Cz_ = comp(C, s.kz);
-    const float Ax = comp(A, s.kx) - s.sx * Az_;
-    const float Ay = comp(A, s.ky) - s.sy * Az_;
-    const float Bx = comp(B, s.kx) - s.sx * Bz_;
-    const float By = comp(B, s.ky) - s.sy * Bz_;
-    const float Cx = comp(C, s.kx) - s.sx * Cz_;
-    const float Cy = comp(C, s.ky) - s.sy * Cz_;
+    // vertices relative to the origin in the permuted frame (kx, ky, kz)
+    const vec3 a(comp(A, s.kx), comp(A, s.ky), comp(A, s.kz));
+    const vec3 b(comp(B, s.kx), comp(B, s.ky), comp(B, s.kz));
+    const vec3 c(comp(C, s.kx), comp(C, s.ky), comp(C, s.kz));
+    const float Az_ = a.z, Bz_ = b.z, Cz_ = c.z;
+    const float Ax = a.x - s.sx * Az_;
+    const float Ay = a.y - s.sy * Az_;
+    const float Bx = b.x - s.sx * Bz_;
+    const float By = b.y - s.sy * Bz_;
+    const float Cx = c.x - s.sx * Cz_;
+    const float Cy = c.y - s.sy * Cz_;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -66,9 +74,16 @@ RM_HD bool intersect_triangle_uvd(vec3 o, const RayShear& s, vec3 v0, vec3 v1, v
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     const float det = (U + V) + W;
     if (det == 0.0f) return false;
-    const float Az = s.sz * Az_, Bz = s.sz * Bz_, Cz = s.sz * Cz_;
-    const float T = (U * Az + V * Bz) + W * Cz;
-    const float tt = T / det;
+    // Distance to the triangle's plane, dot(n, a) / dot(n, d) with
+    // n = (b - a) x (c - a) and d = (sx, sy, 1) / sz in this frame.  A
+    // barycentric blend of the vertices' depths along the ray would carry
+    // rounding of the order of u times the vertices' distance along the ray
+    // (3e-6 for a 1e-4 hit on a 10-unit ground quad at a grazing angle), more
+    // than the box tests' margins; on a plane normal to an axis n has two
+    // exact zeros and this t is within a few roundings of the exact one.
+    const vec3 n = cross(b - a, c - a);
+    const float den = (n.x * s.sx + n.y * s.sy) + n.z;
+    const float tt = (dot(n, a) * s.sz) / den;
     if (!(tt > tmin && tt < tmax)) return false;
     t = tt;
     u_out = U;
@@ -96,8 +111,9 @@ RM_HD bool hit_before(float t, uint32_t inst, uint32_t prim, float bt, uint32_t 
     return prim < bprim;
 }
 
-// 1 + 2*gamma(3) with gamma(n) = n u / (1 - n u), u = 2^-24, rounded up
-constexpr float kBoxSlack = 1.00000048f;
+// 1 + 16u with u = 2^-24: 2*gamma(3) (gamma(n) = n u / (1 - n u)) for the
+// slab test's rounding and 6u for the triangle distance's, rounded up
+constexpr float kBoxSlack = 1.00000095367431640625f;
 
 // Reciprocal direction for the slab test; components below 1e-30 in
 // magnitude are replaced by +-1e-30 so that 0 * inf never produces a NaN.

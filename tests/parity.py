@@ -32,10 +32,29 @@ def assert_radiance_close(got, ref, rtol=RTOL, atol=ATOL, what="radiance"):
     return float((d[lit] / np.abs(ref[lit])).max()) if lit.any() else 0.0
 
 
-def assert_hits_equal(gpu, ref):
+def describe_rays(rays, idx, gpu, ref, limit=8):
+    """The rays at idx as exact float32 bits with both sides' results."""
+    lines = []
+    for i in idx[:limit]:
+        bits = " ".join("%08x" % b for b in np.ascontiguousarray(rays[i], np.float32).view(np.uint32))
+        lines.append("ray %d [%s] gpu t=%r inst=%d prim=%d oracle t=%r inst=%d prim=%d"
+                     % (i, bits, float(gpu[0][i]), int(gpu[1][i]), int(gpu[2][i]), float(ref[0][i]), int(ref[1][i]),
+                        int(ref[2][i])))
+    return "\n".join(lines)
+
+
+def assert_hits_equal(gpu, ref, rays=None):
     t, inst, prim, bary = gpu
     rt, rinst, rprim, rbary = ref
     hit = rt >= 0
+    if rays is not None:
+        both = hit & (t >= 0)
+        bad = np.flatnonzero(((t >= 0) != hit) | (both & ((t.view(np.uint32) != rt.view(np.uint32)) |
+                                                          (inst != rinst) | (prim != rprim) |
+                                                          (bary.view(np.uint32) != rbary.view(np.uint32)).any(1))))
+        if bad.size:
+            raise AssertionError("%d rays differ (%d hit/miss)\n%s" % (bad.size, np.sum((t >= 0) != hit),
+                                                                      describe_rays(rays, bad, gpu, ref)))
     assert np.array_equal(t >= 0, hit), "hit/miss mismatch: %d rays" % np.sum((t >= 0) != hit)
     assert np.array_equal(t.view(np.uint32)[hit], rt.view(np.uint32)[hit]), \
         "distance bits differ on %d rays" % np.sum(t.view(np.uint32)[hit] != rt.view(np.uint32)[hit])

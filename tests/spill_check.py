@@ -50,13 +50,20 @@ def main():
         worst = parity.assert_radiance_close(r.image(), ref, what=name)
         out[name] = dict(bvh_depth=int(depth), rays=int(rays.shape[0]), worst_rel=worst)
         kept.append((name, desc, scene, orc))
-    # a batch keeps its own spill rows for the merged scene
-    jobs = [(scene, desc.width, desc.height, 2, 6, 0) for (_, desc, scene, _) in kept]
+    # a batch keeps its own spill rows for the merged scene; its jobs share
+    # one material library, so the second job is a tessellated Cornell box
+    tess = scenes.tessellated_cornell_box(32, 40, 2000)
+    tess.envmap_resolution = 64
+    tess_orc = binding.OracleScene(tess)
+    tess_orc.build_envmap(threads=parity.ORACLE_THREADS)
+    batch = [kept[0], ("tessellated", tess, scenes.build_device_scene(dev, tess), tess_orc)]
+    jobs = [(scene, desc.width, desc.height, 2, 6, 0) for (_, desc, scene, _) in batch]
     b = api.Batch(dev, jobs)
     b.render(queue_paths=4096)
-    for j, (name, desc, scene, orc) in enumerate(kept):
+    for j, (name, desc, scene, orc) in enumerate(batch):
         ref, _ = orc.render(desc.width, desc.height, 6, frames=2, threads=parity.ORACLE_THREADS)
-        out[name]["batch_worst_rel"] = parity.assert_radiance_close(b.image(j), ref, what="batch " + name)
+        out.setdefault(name, {})["batch_worst_rel"] = parity.assert_radiance_close(b.image(j), ref,
+                                                                                   what="batch " + name)
     print(json.dumps(out), flush=True)
 
 

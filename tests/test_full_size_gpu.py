@@ -32,7 +32,8 @@ import raymond_amd.api as api
 import raymond_amd.scenes as scenes
 from oracle import binding
 from tests import parity
-from tests.test_rcache_cpu import random_features, random_params
+from tests.test_rcache_cpu import numpy_gradients, random_features, random_params
+from tests.test_rcache_gpu import check_gradients
 
 pytestmark = pytest.mark.gpu
 
@@ -89,13 +90,13 @@ def test_field_hits_bit_exact(field):
     g = scene.trace_closest(cam)
     r = orc.trace_closest(cam, threads=T)
     log("camera rays: %d, hit %.3f, oracle %.1f s" % (cam.shape[0], np.mean(r[0] >= 0), time.time() - t0))
-    parity.assert_hits_equal(g, r)
+    parity.assert_hits_equal(g, r, cam)
     t_cam = r[0]
     bounce = parity.bounce_rays(cam, t_cam, seed=22, count=1 << 20)
     g = scene.trace_closest(bounce)
     r = orc.trace_closest(bounce, threads=T)
     log("bounce rays: %d, hit %.3f" % (bounce.shape[0], np.mean(r[0] >= 0)))
-    parity.assert_hits_equal(g, r)
+    parity.assert_hits_equal(g, r, bounce)
     assert np.mean(r[0] >= 0) > 0.3
     rng = np.random.default_rng(23)
     shadow = parity.bounce_rays(cam, t_cam, seed=24, count=1 << 20)
@@ -217,13 +218,10 @@ def test_cache_default_table_matches_oracle(device):
     gm, gt, loss = c.gradients(f[:512], t[:512])
     rm, rt, rloss = binding.rc_gradients(log2, mlp, table, f[:512], t[:512])
     assert abs(loss - rloss) < 1e-2 * rloss
-    for name, sl in (("W1", slice(0, 3072)), ("W2", slice(3072, 7168)), ("W3", slice(7168, 9216))):
-        e = np.linalg.norm(gm[sl] - rm[sl]) / np.linalg.norm(rm[sl])
-        log("2^19 table %s gradient: relative L2 %.3g" % (name, e))
-        assert e < 3e-2, name
-    e = np.linalg.norm(gt - rt) / np.linalg.norm(rt)
-    log("2^19 table gradient: relative L2 %.3g" % e)
-    assert e < 3e-2
+    # with this table the binary16 operands alone put W1 3.4%, W2 3.1% and
+    # the table 4.0% (relative L2) from the double oracle
+    check_gradients(gm, gt, rm, rt, numpy_gradients(log2, mlp, table, f[:512], t[:512], fp16=True),
+                    log=lambda m: log("2^19 table " + m))
 
 
 def test_cache_render_1080p(device):

@@ -187,6 +187,57 @@ def test_oracle_bvh_matches_brute_force():
     assert np.array_equal(orc.trace_any(r), orc.trace_any(r, brute_force=True))
 
 
+# bounce rays leaving the ground of the 8x8 sphere field at grazing angles,
+# whose plane crossing lies just before tmin = 1e-4 (exact float32 bits)
+GRAZING_GROUND_RAYS = """
+c0b3f7ac 3ff1c8c0 35f2d69a 38d1b717 3f6b6c25 3ec8e68e bc961fef 7f800000
+bec5ce4c 40df3544 3598c6b6 38d1b717 bf209073 3f475d56 bc3eb396 7f800000
+3e84ef9d 411036be 350a9d1b 38d1b717 bec70088 bf6bde37 bbac76e3 7f800000
+3f638aa1 40afd8ea 359d6824 38d1b717 3ec0ce6c bf6d229f bc433995 7f800000
+c099dc88 40b78040 35ac7cbe 38d1b717 3e6ae846 bf7926a3 bc5404db 7f800000
+3f4c055c 40a61104 35cc69dc 38d1b717 bf0aa82a 3f57296a bc7d62b7 7f800000
+407edbe0 40bff557 359ff634 38d1b717 3f2a9bc8 3f3ed66c bc47d578 7f800000
+3e7291c3 3f93b9ee 35ee65a4 38d1b717 3f249f96 3f43fee8 bc920065 7f800000
+"""
+
+
+def test_grazing_ground_hits_follow_the_plane():
+    """Rays that cross the ground plane z = 0 near tmin: the hit distance on
+    that axis-aligned plane is within 8 ulp of -o.z / d.z (float64), so hit
+    or miss follows the exact crossing, and the oracle's BVH2 (whose flat
+    ground box clamps at tmin) agrees with the exhaustive loop bit for bit."""
+    desc = scenes.sphere_field(8, 8, grid=8, level=1)
+    orc = binding.OracleScene(desc)
+    fixed = np.array([[int(w, 16) for w in line.split()] for line in GRAZING_GROUND_RAYS.split("\n") if line],
+                     np.uint32).view(np.float32)
+    rng = np.random.default_rng(31)
+    n = 40000
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:2] = rng.uniform(-6, 6, (n, 2))
+    ang = rng.uniform(0, 2 * np.pi, n)
+    dz = -rng.uniform(1e-3, 5e-2, n)
+    d = np.stack([np.cos(ang), np.sin(ang), dz], 1)
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    cross = rng.uniform(0.5e-4, 2e-4, n)  # exact plane crossing wanted
+    r[:, 2] = -cross * r[:, 6].astype(np.float64)
+    r[:, 3] = 1e-4
+    r[:, 7] = np.inf
+    r = np.concatenate([fixed, r])
+    a = orc.trace_closest(r)
+    b = orc.trace_closest(r, brute_force=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    exact = -r[:, 2].astype(np.float64) / r[:, 6].astype(np.float64)
+    ground = len(desc.entities) - 1  # the ground is added last
+    t, inst = b[0], b[1]
+    on_ground = (t >= 0) & (t < 1e-3)
+    assert np.all(np.abs(t[on_ground] - exact[on_ground]) <= 8 * np.spacing(exact[on_ground].astype(np.float32)))
+    clear = np.abs(exact - 1e-4) > 1e-9
+    assert np.array_equal(on_ground[clear], exact[clear] > 1e-4)
+    assert not on_ground[:fixed.shape[0]].any()
+    assert np.all(inst[on_ground] == ground)
+
+
 def test_oracle_counts_and_camera_block_linear(cornell_oracle):
     desc, orc = cornell_oracle
     img, res = orc.render(32, 32, 8, frames=1)

@@ -1,7 +1,9 @@
 """Radiance-cache oracle on the CPU: FP16 rounding, grid geometry and index
 rule against independent numpy restatements, the encoding at grid vertices,
-the forward pass against numpy matrix products, and the analytic gradients
-against central differences of the oracle's own loss."""
+the forward and backward passes against numpy matrix products, and the
+analytic gradients against central differences of the oracle's own loss.
+The numpy restatement also rounds, on request, where the GPU kernels hold
+binary16 values; the GPU tests compare with that."""
 import numpy as np
 import pytest
 
@@ -67,13 +69,18 @@ def random_features(rng, n):
     return f
 
 
-def numpy_forward(log2, mlp, table, f):
-    """Independent restatement: trilinear encoding + three matrix products."""
+def numpy_encode(log2, table, f):
+    """Independent restatement of the input layer in float64: the trilinear
+    hash-grid encoding of 16 levels x 2 features, the 9 further features and
+    a constant 1.  Also returns each (level, corner)'s table entry and weight,
+    shape (16, 8, n), for the table gradient."""
     res, dense = binding.rc_grid(log2)
     T = 2 ** log2
     tab = table.reshape(16, T, 2).astype(np.float64)
     n = f.shape[0]
     x = np.zeros((n, 48))
+    idxs = np.zeros((16, 8, n), np.int64)
+    ws = np.zeros((16, 8, n))
     for l in range(16):
         p = f[:, 0:3].astype(np.float32) * np.float32(res[l])
         c = np.clip(np.floor(p), 0, res[l] - 1)
@@ -89,14 +96,83 @@ def numpy_forward(log2, mlp, table, f):
                 idx = (g[:, 0] ^ ((g[:, 1] * 2654435761) & 0xFFFFFFFF) ^ ((g[:, 2] * 805459861) & 0xFFFFFFFF)) & (T - 1)
             w = np.prod(np.where(b[None, :] == 1, fr, 1.0 - fr), axis=1)
             x[:, 2 * l:2 * l + 2] += w[:, None] * tab[l, idx]
+            idxs[l, k], ws[l, k] = idx, w
     x[:, 32:41] = f[:, 3:12]
     x[:, 41] = 1.0
+    return x, idxs, ws
+
+
+def numpy_weights(mlp):
     W1 = mlp[:3072].reshape(48, 64).astype(np.float64)
     W2 = mlp[3072:7168].reshape(64, 64).astype(np.float64)
     W3 = mlp[7168:].reshape(64, 32)[:, :3].astype(np.float64)
+    return W1, W2, W3
+
+
+def numpy_forward(log2, mlp, table, f):
+    """Independent restatement: trilinear encoding + three matrix products."""
+    W1, W2, W3 = numpy_weights(mlp)
+    x = numpy_encode(log2, table, f)[0]
     h1 = np.maximum(x @ W1, 0)
     h2 = np.maximum(h1 @ W2, 0)
     return h2 @ W3
+
+
+def to_half(a):
+    return np.asarray(a).astype(np.float16).astype(np.float64)
+
+
+def numpy_gradients(log2, mlp, table, f, t, fp16=False):
+    """Mean relative-L2 loss sum_c (y - t)^2 / (y^2 + 0.01) and its gradients
+    by numpy matrix products, laid out as the oracle's (mlp, table) arrays.
+
+    With fp16 the values that the GPU kernels hold in binary16 are rounded
+    where they do: the encoded inputs, both hidden activations, dY, dZ2 and
+    dZ1 (the MFMA operands, products summed exactly), and each table-gradient
+    contribution, scaled by 4096 / batch, clamped to +-2048 and summed in
+    binary16 in sample order.  Samples with a non-finite target contribute
+    nothing."""
+    W1, W2, W3 = numpy_weights(mlp)
+    r = to_half if fp16 else (lambda a: a)
+    n = f.shape[0]
+    x, idxs, ws = numpy_encode(log2, table, f)
+    x = r(x)
+    h1 = r(np.maximum(x @ W1, 0))
+    h2 = r(np.maximum(h1 @ W2, 0))
+    y = h2 @ W3
+    t = np.asarray(t, np.float64)
+    ok = np.all(np.isfinite(t), axis=1)
+    tt = np.where(ok[:, None], t, 0.0)
+    den = y * y + 0.01
+    loss = float(np.sum(np.where(ok[:, None], (y - tt) ** 2 / den, 0.0)) / n)
+    dy = r(np.where(ok[:, None], 2.0 * (y - tt) / den, 0.0))
+    dz2 = r(dy @ W3.T) * (h2 > 0)
+    dz1 = r(dz2 @ W2.T) * (h1 > 0)
+    gm = np.zeros(9216)
+    gm[:3072] = (x.T @ dz1).ravel() / n
+    gm[3072:7168] = (h1.T @ dz2).ravel() / n
+    g3 = np.zeros((64, 32))
+    g3[:, :3] = h2.T @ dy / n
+    gm[7168:] = g3.ravel()
+    dx = dz1 @ W1[:32].T  # encoding features
+    T = 2 ** log2
+    if fp16:
+        acc = np.zeros(16 * T * 2, np.float16)
+        scale = np.float32(1.0 / n) * np.float32(4096.0)
+        for l in range(16):
+            for c in range(2):
+                g = (dx[:, 2 * l + c] * scale).astype(np.float32)
+                for k in range(8):
+                    v = np.clip(ws[l, k].astype(np.float32) * g, -2048.0, 2048.0).astype(np.float16)
+                    np.add.at(acc, (l * T + idxs[l, k]) * 2 + c, v)
+        gt = acc.astype(np.float64) / 4096.0
+    else:
+        gt = np.zeros(16 * T * 2)
+        for l in range(16):
+            for c in range(2):
+                for k in range(8):
+                    np.add.at(gt, (l * T + idxs[l, k]) * 2 + c, ws[l, k] * dx[:, 2 * l + c] / n)
+    return gm, gt, loss
 
 
 def test_encoding_at_grid_vertices():
@@ -134,6 +210,26 @@ def test_forward_matches_numpy():
     out = binding.rc_forward(LOG2, mlp, table, f)
     ref = numpy_forward(LOG2, mlp, table, f)
     assert np.allclose(out, ref, rtol=1e-9, atol=1e-9)
+
+
+def test_gradients_match_numpy():
+    """The oracle's backward pass against the numpy restatement (double),
+    with an unusable sample; and the binary16 restatement's distance from it
+    stays at the size that the binary16 operands explain."""
+    rng = np.random.default_rng(6)
+    mlp, table = random_params(rng, LOG2)
+    f = random_features(rng, 300)
+    y = binding.rc_forward(LOG2, mlp, table, f)
+    t = (y + rng.normal(size=y.shape) * 0.5).astype(np.float32)
+    t[7] = np.nan
+    gm, gt, loss = binding.rc_gradients(LOG2, mlp, table, f, t)
+    nm, nt, nloss = numpy_gradients(LOG2, mlp, table, f, t)
+    assert abs(loss - nloss) < 1e-12 * nloss
+    assert np.allclose(gm, nm, rtol=1e-9, atol=1e-12 * np.abs(nm).max())
+    assert np.allclose(gt, nt, rtol=1e-9, atol=1e-12 * np.abs(nt).max())
+    hm, ht, _ = numpy_gradients(LOG2, mlp, table, f, t, fp16=True)
+    e = np.linalg.norm(hm - nm) / np.linalg.norm(nm)
+    assert 1e-4 < e < 0.1
 
 
 def test_gradients_match_central_differences():

@@ -7,7 +7,7 @@ import pytest
 import raymond_amd.api as api
 import raymond_amd.scenes as scenes
 from oracle import binding
-from tests.test_rcache_cpu import random_features, random_params
+from tests.test_rcache_cpu import numpy_gradients, random_features, random_params
 
 pytestmark = pytest.mark.gpu
 
@@ -51,10 +51,24 @@ def test_gradients_match_oracle(cache):
     w3[:, :3] = True
     w[7168:] = w3.ravel()
     assert np.all(gm[~w] == 0.0)
-    for name, sl in (("W1", slice(0, 3072)), ("W2", slice(3072, 7168)), ("W3", slice(7168, 9216))):
-        assert rel_l2(gm[sl], rm[sl]) < 3e-2, name
-    assert rel_l2(gt, rt) < 3e-2
+    check_gradients(gm, gt, rm, rt, numpy_gradients(LOG2, mlp, table, f, t, fp16=True))
     assert np.array_equal(gt != 0.0, rt != 0.0) or (np.sum((gt != 0) != (rt != 0)) < 1e-3 * gt.size)
+
+
+def check_gradients(gm, gt, rm, rt, half, log=print):
+    """GPU gradients against the numpy restatement that rounds where the
+    kernels hold binary16 values (bound 5e-3: accumulation order and the odd
+    binary16 rounding that float32 and float64 sums put on different sides),
+    and against the double oracle within that restatement's own distance
+    from it plus the same 5e-3.  Relative L2 per block."""
+    hm, ht, _ = half
+    blocks = (("W1", slice(0, 3072)), ("W2", slice(3072, 7168)), ("W3", slice(7168, 9216)))
+    for name, got, half_ref, ref in [(n, gm[sl], hm[sl], rm[sl]) for n, sl in blocks] + [("table", gt, ht, rt)]:
+        e_half, e_ref, e_model = rel_l2(got, half_ref), rel_l2(got, ref), rel_l2(half_ref, ref)
+        log("%s gradient: relative L2 %.3g to the binary16 restatement, %.3g to the oracle (restatement %.3g)"
+            % (name, e_half, e_ref, e_model))
+        assert e_half < 5e-3, name
+        assert e_ref < e_model + 5e-3, name
 
 
 def test_adam_first_step_closed_form(cache):
