@@ -262,6 +262,38 @@ void rm_renderer_destroy(rm_renderer* r);
 int rm_renderer_set_uniforms(rm_renderer* r, const rm_uniforms* u);
 /* Clears the accumulation and the frame counter (Renderer.reset). */
 int rm_renderer_reset(rm_renderer* r);
+
+/* A lens of the camera (the reference's lore::Surface array, loaded by
+ * io/LensLoader.mm:17-42).  Surfaces from the object side to the image plane
+ * (the last one): signed radius of the sphere cap (0: plane) with its vertex
+ * on the axis, distance to the next surface toward the sensor, aperture
+ * radius (mm), and the medium after the surface toward the sensor by n_d and
+ * Abbe number V_d (0: no dispersion). */
+typedef struct rm_lens_surface {
+    float radius, thickness, aperture, ior_d, abbe;
+} rm_lens_surface;
+
+/* Lens settings (bridge/Uniforms.hpp:34-41; defaults Renderer.swift:333-343:
+ * spectral 1, sensor_scale 1, camera_scale 0.001, focus 0, stop_index 0,
+ * relative_stop 1, aperture_blades 7). */
+typedef struct rm_lens_desc {
+    uint32_t num_surfaces;           /* 0: the pinhole camera */
+    const rm_lens_surface* surfaces;
+    int32_t stop_index;              /* surface with the aperture blades (0: none) */
+    float relative_stop;             /* opening of the stop, 0..1 */
+    int32_t aperture_blades;
+    float sensor_scale;              /* sensor size relative to 36 mm film */
+    float camera_scale;              /* lens millimetres to scene units */
+    float focus;                     /* sensor position along the axis (mm) */
+    uint32_t spectral;               /* one wavelength per path, weighted to RGB */
+} rm_lens_desc;
+
+/* Renderer.setLens (Renderer.swift:738-746): camera rays start on the
+ * sensor, aim at the rear element and are traced through the lens
+ * (generate.metal:122-163); a ray the lens stops contributes nothing.
+ * NULL or num_surfaces 0 restores the pinhole camera.  The surfaces are
+ * copied. */
+int rm_renderer_set_lens(rm_renderer* r, const rm_lens_desc* lens);
 /* Renders `samples` more samples per pixel (Renderer.execute) as one
  * regenerating stream: the path queue holds width*height*samples_per_launch
  * paths and is refilled with new camera paths as paths terminate.  Blocks

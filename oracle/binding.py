@@ -29,7 +29,7 @@ class RenderArgs(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_uint32),
                 ("sampling_mode", C.c_uint32), ("rr_depth", C.c_uint32), ("frame_begin", C.c_uint32),
                 ("frames", C.c_uint32), ("pixels", C.POINTER(C.c_uint32)), ("num_pixels", C.c_uint32),
-                ("threads", C.c_int), ("brute_force", C.c_int)]
+                ("threads", C.c_int), ("brute_force", C.c_int), ("lens", C.c_void_p)]
 
 
 class RenderResult(C.Structure):
@@ -143,11 +143,12 @@ class OracleScene:
         return occ
 
     def render(self, width, height, max_depth, frames=1, frame_begin=0, sampling_mode=2, rr_depth=0,
-               pixels=None, threads=0, brute_force=False, image=None):
+               pixels=None, threads=0, brute_force=False, image=None, lens=None):
+        """lens: an rm_lens_desc structure (raymond_amd.api.lens_desc) or None."""
         if image is None:
             image = np.zeros((height, width, 4), np.float32)
         args = RenderArgs(width, height, max_depth, sampling_mode, rr_depth, frame_begin, frames, None, 0,
-                          threads, int(brute_force))
+                          threads, int(brute_force), C.addressof(lens) if lens is not None else None)
         if pixels is not None:
             pixels = np.ascontiguousarray(pixels, np.uint32)
             args.pixels = pixels.ctypes.data_as(C.POINTER(C.c_uint32))

@@ -18,6 +18,7 @@
 
 #include "../raymond_amd/csrc/device/isect.h"
 #include "../raymond_amd/csrc/device/material_prelude.h"
+#include "../raymond_amd/csrc/device/camera.h"
 
 namespace {
 
@@ -375,8 +376,8 @@ struct PathCounts {
 
 // generateRays + depth loop of handleIntersections / handleShadowRays for one path
 // px: the RGBA sums of pixel (x, y)
-void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_t y, uint32_t frame, float* px,
-                 PathCounts& cnt) {
+void render_path(const or_scene& S, const or_render_args& A, const rm::Lens& lens, uint32_t x, uint32_t y,
+                 uint32_t frame, float* px, PathCounts& cnt) {
     using namespace rm;
     const SceneView& ctx = S.view;
     const uint32_t W = A.width, H = A.height;
@@ -391,14 +392,20 @@ void render_path(const or_scene& S, const or_render_args& A, uint32_t x, uint32_
     ray.bsdf_pdf = kInf;
     ray.x = x;
     ray.y = y;
-    const vec2 jr = ray.prng.sample2d();
-    const vec2 jittered((float)x + jr.x, (float)y + jr.y);
-    const vec2 uv(+1.0f * ((jittered.x / (float)W + ctx.camera.shift.x) * 2.0f - 1.0f),
-                  -1.0f * ((jittered.y / (float)H + ctx.camera.shift.y) * 2.0f - 1.0f));
-    const float aspect = (float)H / (float)W;
-    ray.origin = ctx.camera.transform.point(vec3(0.0f, 0.0f, 0.0f));
-    ray.direction = normalize(ctx.camera.transform.vector(vec3(uv.x, uv.y * aspect, -ctx.camera.focal_length)));
-    ray.weight = vec3(1.0f);
+    if (lens.num_surfaces > 0) {
+        // a ray the lens stops is never committed (generate.metal:143-146)
+        if (!rm::lens_camera_ray(ctx.camera, lens, x, y, W, H, ray.prng, ray.origin, ray.direction, ray.weight))
+            return;
+    } else {
+        const vec2 jr = ray.prng.sample2d();
+        const vec2 jittered((float)x + jr.x, (float)y + jr.y);
+        const vec2 uv(+1.0f * ((jittered.x / (float)W + ctx.camera.shift.x) * 2.0f - 1.0f),
+                      -1.0f * ((jittered.y / (float)H + ctx.camera.shift.y) * 2.0f - 1.0f));
+        const float aspect = (float)H / (float)W;
+        ray.origin = ctx.camera.transform.point(vec3(0.0f, 0.0f, 0.0f));
+        ray.direction = normalize(ctx.camera.transform.vector(vec3(uv.x, uv.y * aspect, -ctx.camera.focal_length)));
+        ray.weight = vec3(1.0f);
+    }
     Hit isect = trace(S, ray.origin, ray.direction, ray.min_distance, ray.max_distance, false, A.brute_force != 0);
 
     for (uint32_t depth = 0; depth < A.max_depth; depth++) {
@@ -815,6 +822,23 @@ static int render_impl(or_scene* s, const or_render_args* A, float* image, doubl
     if (nt < 1) nt = 1;
     std::vector<PathCounts> counts(nt);
     const MatTable* mt = &s->mat;
+    rm::Lens lens{};
+    std::vector<rm::LensSurface> lens_surfaces;
+    if (A->lens && A->lens->num_surfaces > 0) {
+        for (uint32_t i = 0; i < A->lens->num_surfaces; i++) {
+            const rm_lens_surface& a = A->lens->surfaces[i];
+            lens_surfaces.push_back(rm::LensSurface{a.radius, a.thickness, a.aperture, a.ior_d, a.abbe});
+        }
+        lens.surfaces = lens_surfaces.data();
+        lens.num_surfaces = (int)A->lens->num_surfaces;
+        lens.stop_index = A->lens->stop_index;
+        lens.relative_stop = A->lens->relative_stop;
+        lens.aperture_blades = A->lens->aperture_blades;
+        lens.sensor_scale = A->lens->sensor_scale;
+        lens.camera_scale = A->lens->camera_scale;
+        lens.focus = A->lens->focus;
+        lens.spectral = A->lens->spectral ? 1 : 0;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     // full images go in 8x8 pixel tiles, raster order of tiles
     const uint32_t tiles_x = (A->width + 7) / 8, tiles_y = (A->height + 7) / 8;
@@ -834,12 +858,12 @@ static int render_impl(or_scene* s, const or_render_args* A, float* image, doubl
         const size_t at = 4 * ((size_t)y * A->width + x);
         for (uint32_t f = 0; f < A->frames; f++) {
             if (image) {
-                render_path(*s, *A, x, y, A->frame_begin + f, image + at, counts[th]);
+                render_path(*s, *A, lens, x, y, A->frame_begin + f, image + at, counts[th]);
             } else {
                 // each sample summed in FP32 (its contributions in path order),
                 // the samples in FP64
                 float px[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                render_path(*s, *A, x, y, A->frame_begin + f, px, counts[th]);
+                render_path(*s, *A, lens, x, y, A->frame_begin + f, px, counts[th]);
                 for (int c = 0; c < 4; c++) image64[at + c] += (double)px[c];
             }
         }

@@ -64,6 +64,7 @@ typedef struct or_render_args {
     uint32_t num_pixels;
     int threads;
     int brute_force;
+    const rm_lens_desc* lens;  /* optional: camera rays through this lens (generate.metal:122-163) */
 } or_render_args;
 
 typedef struct or_render_result {

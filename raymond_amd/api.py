@@ -68,6 +68,30 @@ class TextureDesc(C.Structure):
                 ("pixels", C.POINTER(C.c_float))]
 
 
+class LensSurface(C.Structure):
+    _fields_ = [("radius", C.c_float), ("thickness", C.c_float), ("aperture", C.c_float), ("ior_d", C.c_float),
+                ("abbe", C.c_float)]
+
+
+class LensDesc(C.Structure):
+    _fields_ = [("num_surfaces", C.c_uint32), ("surfaces", C.POINTER(LensSurface)), ("stop_index", C.c_int32),
+                ("relative_stop", C.c_float), ("aperture_blades", C.c_int32), ("sensor_scale", C.c_float),
+                ("camera_scale", C.c_float), ("focus", C.c_float), ("spectral", C.c_uint32)]
+
+
+def lens_desc(surfaces, stop_index=0, relative_stop=1.0, aperture_blades=7, sensor_scale=1.0, camera_scale=0.001,
+              focus=0.0, spectral=True):
+    """rm_lens_desc from surfaces (n, 5): radius, thickness, aperture, n_d,
+    V_d, object side first, the image plane last (defaults as the
+    reference's uniforms, Renderer.swift:333-343)."""
+    s = np.ascontiguousarray(surfaces, np.float32).reshape(-1, 5)
+    arr = (LensSurface * s.shape[0])(*[LensSurface(*map(float, row)) for row in s])
+    d = LensDesc(s.shape[0], arr, stop_index, relative_stop, aperture_blades, sensor_scale, camera_scale, focus,
+                 int(bool(spectral)))
+    d._keep = arr
+    return d
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("num_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
@@ -171,6 +195,7 @@ EXPORTS = {
     "rm_renderer_destroy": (None, [C.c_void_p]),
     "rm_renderer_set_uniforms": (C.c_int, [C.c_void_p, C.POINTER(Uniforms)]),
     "rm_renderer_reset": (C.c_int, [C.c_void_p]),
+    "rm_renderer_set_lens": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rm_render": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(RenderStats)]),
     "rm_read_image": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "rm_read_image_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -477,6 +502,11 @@ class Renderer:
 
     def reset(self):
         _check(LIB.rm_renderer_reset(self.handle))
+
+    def set_lens(self, lens):
+        """Renderer.setLens: an api.lens_desc(...) or None for the pinhole camera."""
+        self._lens = lens
+        _check(LIB.rm_renderer_set_lens(self.handle, C.byref(lens) if lens is not None else None))
 
     def execute(self, samples=1):
         st = RenderStats()

@@ -8,6 +8,7 @@
 #include "elem.h"
 #include "noise.h"
 #include "shading.h"
+#include "spectral.h"
 #include "texture.h"
 #include "warp.h"
 
@@ -581,12 +582,6 @@ RM_HD int floor_to_int(float x) {
 RM_HD float srgb_to_linear(float c) {
     if (c < 0.04045f) return c < 0.0f ? 0.0f : c * (1.0f / 12.92f);
     return pow_((c + 0.055f) * (1.0f / 1.055f), 2.4f);
-}
-// CIE XYZ to linear sRGB primaries, D65 (IEC 61966-2-1)
-RM_HD vec3 xyz_to_rgb(vec3 v) {
-    return vec3((3.240479f * v.x + -1.537150f * v.y) + -0.498535f * v.z,
-                (-0.969256f * v.x + 1.875991f * v.y) + 0.041556f * v.z,
-                (0.055648f * v.x + -0.204043f * v.y) + 1.057311f * v.z);
 }
 
 struct kTexImage {

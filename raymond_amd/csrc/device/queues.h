@@ -11,6 +11,9 @@
 namespace rm {
 
 constexpr int kMaxDepth = 32;
+// depth of a camera path that carries no ray: its lens trace failed, and the
+// reference commits no ray for it (generate.metal:143-146)
+constexpr uint32_t kVoidDepth = 255;
 
 // path meta word: RNG draw index (16) | ray flags (8) | depth (8)
 RM_HD uint32_t pack_meta(uint32_t prng_index, uint32_t flags, uint32_t depth) {
@@ -57,7 +60,8 @@ struct IterCounters {
     uint32_t shadows;        // shadow rays emitted by shade(i)
     uint32_t fetch_closest;  // persistent-scheduler counters
     uint32_t fetch_any;
-    uint32_t pad0, pad1;
+    uint32_t voids;          // new camera paths without a ray (counted in queue, not traced)
+    uint32_t pad1;
 };
 
 constexpr int kMaxIters = 2048;

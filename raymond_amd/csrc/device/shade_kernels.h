@@ -170,6 +170,8 @@ __device__ inline __attribute__((always_inline)) void shade_path(const ShadeArgs
     vec3 rad(0.0f);
     if (training) rad = vec3(slot_rad[0], slot_rad[1], slot_rad[2]);
     out.slot = slot;
+    // a camera path whose lens trace failed carries no ray (generate.metal:143-146)
+    if (depth == kVoidDepth) return;
     auto contribute = [&](vec3 c) {
         image_add(a.image, pixel, c);
         if (training) rad += c;

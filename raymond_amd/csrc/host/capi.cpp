@@ -246,6 +246,13 @@ int rm_renderer_set_uniforms(rm_renderer* r, const rm_uniforms* u) {
     return RM_OK;
 }
 
+int rm_renderer_set_lens(rm_renderer* r, const rm_lens_desc* lens) {
+    RM_NULL_CHECK(r);
+    RM_GUARD_BEGIN
+    rm::renderer_set_lens(r, lens);
+    RM_GUARD_END
+}
+
 int rm_renderer_reset(rm_renderer* r) {
     RM_NULL_CHECK(r);
     RM_GUARD_BEGIN

@@ -13,6 +13,7 @@
 #include "../device/rcache.h"
 #include "../device/batch.h"
 #include "../device/scene.h"
+#include "../device/camera.h"
 
 namespace rm {
 
@@ -147,6 +148,8 @@ struct rm_renderer {
     rm_uniforms u{};
     uint32_t frame_index = 0;  // samples accumulated so far
     uint32_t last_samples = 0; // samples of the last rm_render call (the image's content when not accumulating)
+    rm::Lens lens{};           // num_surfaces 0: pinhole
+    rm::DevBuf lens_buf;
     std::vector<hipEvent_t> events;
     // radiance cache (rm_renderer_set_cache)
     rm_cache* cache = nullptr;

@@ -71,6 +71,38 @@ void renderer_init(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, size
 
 static void join_cache_work(rm_renderer* r);
 
+void renderer_set_lens(rm_renderer* r, const rm_lens_desc* d) {
+    if (!d || d->num_surfaces == 0) {
+        r->lens = Lens{};
+        return;
+    }
+    RM_REQUIRE(d->surfaces != nullptr, "lens surfaces missing");
+    RM_REQUIRE(d->num_surfaces >= 2, "a lens needs a rear element and the image plane");
+    RM_REQUIRE(d->stop_index >= 0 && d->stop_index < (int32_t)d->num_surfaces, "lens stop index out of range");
+    RM_REQUIRE(d->aperture_blades >= 3, "an aperture needs at least 3 blades");
+    RM_REQUIRE(d->relative_stop > 0.0f && d->relative_stop <= 1.0f, "relative stop must be in (0, 1]");
+    std::vector<LensSurface> s(d->num_surfaces);
+    for (uint32_t i = 0; i < d->num_surfaces; i++) {
+        const rm_lens_surface& a = d->surfaces[i];
+        RM_REQUIRE(a.aperture > 0.0f && a.ior_d > 0.0f, "lens surface " + std::to_string(i) +
+                                                           " needs a positive aperture and index");
+        s[i] = LensSurface{a.radius, a.thickness, a.aperture, a.ior_d, a.abbe};
+    }
+    hipStream_t st = r->scene->dev->stream;
+    r->lens_buf.upload(s.data(), s.size() * sizeof(LensSurface), st);
+    RM_HIP(hipStreamSynchronize(st));
+    Lens& L = r->lens;
+    L.surfaces = r->lens_buf.as<LensSurface>();
+    L.num_surfaces = (int)d->num_surfaces;
+    L.stop_index = d->stop_index;
+    L.relative_stop = d->relative_stop;
+    L.aperture_blades = d->aperture_blades;
+    L.sensor_scale = d->sensor_scale;
+    L.camera_scale = d->camera_scale;
+    L.focus = d->focus;
+    L.spectral = d->spectral ? 1 : 0;
+}
+
 void renderer_reset(rm_renderer* r) {
     hipStream_t st = r->scene->dev->stream;
     // pending cache lookups of the last stream add into the image
@@ -265,6 +297,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     fu.image_offset = 0;
 
     RegenArgs ga{};
+    ga.lens = br ? Lens{} : r->lens;
     ga.counters = cnt;
     ga.capacity = (uint32_t)r->capacity;
     ga.total = total;
@@ -548,10 +581,12 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     r->frame_index += samples;
     if (stats) {
         for (uint32_t i = 0; i < iter; i++) {
-            if (i < 33) stats->rays[i] += host_it[i].queue;
+            // paths the lens stopped hold a queue entry but no ray
+            const uint32_t traced = host_it[i].queue - host_it[i].voids;
+            if (i < 33) stats->rays[i] += traced;
             if (i < 32) stats->shadow_rays[i] += host_it[i].shadows;
-            stats->total_rays += (uint64_t)host_it[i].queue + host_it[i].shadows;
-            stats->closest_total += host_it[i].queue;
+            stats->total_rays += (uint64_t)traced + host_it[i].shadows;
+            stats->closest_total += traced;
             stats->shadow_total += host_it[i].shadows;
         }
         stats->samples += (uint64_t)total;

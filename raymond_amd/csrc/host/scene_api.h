@@ -29,6 +29,7 @@ void renderer_init(rm_scene* s, uint32_t w, uint32_t h, uint32_t max_depth, size
 // One bounce loop over the slices of `br` (rm_batch_render).
 void renderer_render_batch_loop(rm_renderer* r, BatchRun* br, rm_render_stats* stats);
 void renderer_reset(rm_renderer* r);
+void renderer_set_lens(rm_renderer* r, const rm_lens_desc* lens);
 void renderer_render(rm_renderer* r, uint32_t samples, rm_render_stats* stats);
 void renderer_read_image(rm_renderer* r, float* rgba, size_t count);
 void renderer_read_image_f64(rm_renderer* r, double* rgba, size_t count);

@@ -374,12 +374,11 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
             if (a.tmin_arr) {
                 tmin = a.tmin_arr[i];
                 tmax = a.tmax_arr[i];
-            } else if (meta_depth(q.meta[i]) == 0) {
-                tmin = near_clip;
-                tmax = far_clip;
             } else {
-                tmin = kEps;
-                tmax = kInf;
+                const uint32_t depth = meta_depth(q.meta[i]);
+                // a void path's empty range ends its traversal at the root
+                tmin = depth == 0 ? near_clip : (depth == kVoidDepth ? 1.0f : kEps);
+                tmax = depth == 0 ? far_clip : (depth == kVoidDepth ? 0.0f : kInf);
             }
             s.init(root, o, d, tmin, tmax);
             active = true;
@@ -445,16 +444,29 @@ __global__ void regenerate_kernel(RegenArgs a) {
         uint32_t x, y;
         block_linear_to_pixel(ray_index, width, height, x, y);
         Prng prng(seed + s, ray_index);
-        vec3 o, d;
-        camera_ray(cam, x, y, width, height, prng, o, d);
+        vec3 o, d, w(1.0f);
+        bool ray = true;
+        if (a.lens.num_surfaces > 0) ray = lens_camera_ray(cam, a.lens, x, y, width, height, prng, o, d, w);
+        else camera_ray(cam, x, y, width, height, prng, o, d);
+        if (a.lens.num_surfaces > 0) {
+            // one count per wave of the paths that carry no ray
+            const unsigned long long v = __ballot(!ray);
+            if (v && (uint32_t)__lane_id() == (uint32_t)__builtin_ctzll(__ballot(true)))
+                atomicAdd(&a.counters->it[iter].voids, (uint32_t)__popcll(v));
+            if (!ray) {
+                o = vec3(0.0f);
+                d = vec3(0.0f, 0.0f, 1.0f);
+                w = vec3(0.0f);
+            }
+        }
         const uint32_t j = appended + t;
         PathQueue& q = a.paths;
         q.ox[j] = o.x; q.oy[j] = o.y; q.oz[j] = o.z;
         q.dx[j] = d.x; q.dy[j] = d.y; q.dz[j] = d.z;
-        q.wx[j] = 1.0f; q.wy[j] = 1.0f; q.wz[j] = 1.0f;
+        q.wx[j] = w.x; q.wy[j] = w.y; q.wz[j] = w.z;
         q.pdf[j] = kInf;
         q.seed[j] = prng.seed;
-        q.meta[j] = pack_meta(prng.index, RayCamera, 0);
+        q.meta[j] = pack_meta(prng.index, RayCamera, ray ? 0u : kVoidDepth);
         q.pixel[j] = image_offset + y * width + x;
         if (a.slots) {
             const uint32_t slot = rc_training_slot(g, a.random_seed, a.train_shift);

@@ -5,6 +5,7 @@
 #include "../device/queues.h"
 #include "../device/scene.h"
 #include "../device/batch.h"
+#include "../device/camera.h"
 
 namespace rm {
 
@@ -40,6 +41,7 @@ struct ClosestArgs {
 };
 
 struct RegenArgs {
+    Lens lens;                  // num_surfaces 0: pinhole camera
     Counters* counters;
     uint32_t iter;
     const uint32_t* iter_dev;   // captured bounce loops (iteration_of)

@@ -631,6 +631,21 @@ def sphere_field(width=1920, height=1080, grid=8, level=6, radius=0.4, amplitude
     return d
 
 
+def singlet_lens(focal_mm=50.0, f_number=4.0):
+    """A biconvex N-BK7 singlet (n_d 1.5168, V_d 64.17) with a stop behind it,
+    as lens surfaces for api.lens_desc: object medium, front and back faces,
+    stop, image plane.  Equal radii R = 2 f (n - 1) give the thin-lens focal
+    length f; the stop sits 5 mm behind the lens, the image plane at f."""
+    n = 1.5168
+    R = 2.0 * focal_mm * (n - 1.0)
+    ap = focal_mm / (2.0 * f_number)
+    return np.array([[0.0, 0.0, 100.0, 1.0, 0.0],
+                     [R, 4.0, ap * 1.2, n, 64.17],
+                     [-R, 5.0, ap * 1.2, 1.0, 0.0],
+                     [0.0, focal_mm - 5.0, ap, 1.0, 0.0],
+                     [0.0, 0.0, 100.0, 1.0, 0.0]], np.float32)
+
+
 def build_device_scene(device, desc, threads=0, envmap=True, noinline=None):
     """SceneLoader.loadScene: materials (JIT), shapes/entities (upload + BVH), lights (envmap)."""
     scene = api.Scene(device, desc)

@@ -348,3 +348,73 @@ def sky_nishita(pixels, data, dirs, L=None):
     out = np.zeros((dirs.shape[0], 3), np.float32)
     L.shim_sky_nishita(_p(px), px.shape[1], px.shape[0], _p(data), _p(dirs), _p(out), dirs.shape[0])
     return out
+
+
+# lens camera (camera.h): settings = (stop_index, relative_stop, blades,
+# sensor_scale, camera_scale, focus, spectral)
+def lens_settings(stop_index=0, relative_stop=1.0, blades=7, sensor_scale=1.0, camera_scale=1.0, focus=0.0,
+                  spectral=0):
+    return f32([stop_index, relative_stop, blades, sensor_scale, camera_scale, focus, spectral])
+
+
+def lens_aperture(pos, relative_stop, blades, L=None):
+    L = L or lib()
+    pos = f32(pos).reshape(-1, 2)
+    out = np.zeros(pos.shape[0], np.uint32)
+    L.shim_lens_aperture(_p(pos), pos.shape[0], C.c_float(relative_stop), blades, _p(out))
+    return out.astype(bool)
+
+
+def lens_refract(d, n, eta, L=None):
+    L = L or lib()
+    d, n = f32(d).reshape(-1, 3), f32(n).reshape(-1, 3)
+    eta = f32(np.broadcast_to(eta, (d.shape[0],)))
+    out = np.zeros_like(d)
+    ok = np.zeros(d.shape[0], np.uint32)
+    L.shim_lens_refract(_p(d), _p(n), _p(eta), d.shape[0], _p(out), _p(ok))
+    return out, ok.astype(bool)
+
+
+def lens_ior(ior_d, abbe, lam, L=None):
+    L = L or lib()
+    lam = f32(lam).ravel()
+    out = np.zeros_like(lam)
+    L.shim_lens_ior(C.c_float(ior_d), C.c_float(abbe), _p(lam), _p(out), lam.size)
+    return out
+
+
+def lens_trace(surfaces, settings, o, d, lam, L=None):
+    L = L or lib()
+    s, st = f32(surfaces).reshape(-1, 5), f32(settings)
+    o, d = f32(o).reshape(-1, 3), f32(d).reshape(-1, 3)
+    lam = f32(np.broadcast_to(lam, (o.shape[0],)))
+    oo, dd = np.zeros_like(o), np.zeros_like(d)
+    ok = np.zeros(o.shape[0], np.uint32)
+    L.shim_lens_trace(_p(s), s.shape[0], _p(st), _p(o), _p(d), _p(lam), _p(oo), _p(dd), _p(ok), o.shape[0])
+    return oo, dd, ok.astype(bool)
+
+
+def lens_camera_ray(surfaces, settings, W, H, x, y, seed, n, L=None):
+    L = L or lib()
+    s, st = f32(surfaces).reshape(-1, 5), f32(settings)
+    o, d, w = (np.zeros((n, 3), np.float32) for _ in range(3))
+    ok = np.zeros(n, np.uint32)
+    L.shim_lens_camera_ray(_p(s), s.shape[0], _p(st), W, H, x, y, seed, _p(o), _p(d), _p(w), _p(ok), n)
+    return o, d, w, ok.astype(bool)
+
+
+def spectral(nm, L=None):
+    L = L or lib()
+    nm = f32(nm).ravel()
+    xyz, rgb = np.zeros((nm.size, 3), np.float32), np.zeros((nm.size, 3), np.float32)
+    norm = np.zeros(3, np.float32)
+    L.shim_spectral(_p(nm), nm.size, _p(xyz), _p(rgb), _p(norm))
+    return xyz, rgb, norm
+
+
+def xyz_to_rgb(xyz, L=None):
+    L = L or lib()
+    xyz = f32(xyz).reshape(-1, 3)
+    out = np.zeros_like(xyz)
+    L.shim_xyz_to_rgb(_p(xyz), _p(out), xyz.shape[0])
+    return out

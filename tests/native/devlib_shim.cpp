@@ -1,6 +1,7 @@
 // Host build of the shared device library for CPU property tests: exposes
-// RNG, elementary functions, warps, BSDF lobes, lights and the triangle test
-// through plain C entry points.  Test infrastructure only.
+// RNG, elementary functions, warps, BSDF lobes, lights, the triangle test and
+// the lens camera through plain C entry points.  Test infrastructure only.
+#include "camera.h"
 #include "isect.h"
 #include "material_prelude.h"
 
@@ -486,6 +487,100 @@ void shim_sky_nishita(const float* pixels, int w, int h, const float* data, cons
     for (int i = 0; i < n; i++) {
         const vec3 c = sky_radiance_nishita(vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), data, t);
         out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lens camera.  A lens is passed as `ns` rows of (radius, thickness,
+// aperture, n_d, V_d) and settings[] = {stop_index, relative_stop, blades,
+// sensor_scale, camera_scale, focus, spectral}.
+
+static Lens make_lens(const float* surfaces, int ns, const float* settings) {
+    Lens L{};
+    L.surfaces = reinterpret_cast<const LensSurface*>(surfaces);
+    L.num_surfaces = ns;
+    L.stop_index = (int)settings[0];
+    L.relative_stop = settings[1];
+    L.aperture_blades = (int)settings[2];
+    L.sensor_scale = settings[3];
+    L.camera_scale = settings[4];
+    L.focus = settings[5];
+    L.spectral = (int)settings[6];
+    return L;
+}
+
+// the stop's opening at positions relative to its aperture radius
+void shim_lens_aperture(const float* pos, int n, float relative_stop, int blades, uint32_t* inside) {
+    Lens L{};
+    L.relative_stop = relative_stop;
+    L.aperture_blades = blades;
+    for (int i = 0; i < n; i++) inside[i] = lens_aperture(vec2(pos[2 * i], pos[2 * i + 1]), L);
+}
+
+void shim_lens_refract(const float* d, const float* nrm, const float* eta, int n, float* out, uint32_t* ok) {
+    for (int i = 0; i < n; i++) {
+        vec3 v(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        ok[i] = lens_refract(v, vec3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), eta[i]);
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+
+// n_d and V_d of one surface at wavelengths l (micrometres)
+void shim_lens_ior(float ior_d, float abbe, const float* l, float* out, int n) {
+    const LensSurface s{0.0f, 0.0f, 1.0f, ior_d, abbe};
+    for (int i = 0; i < n; i++) out[i] = lens_ior(s, l[i]);
+}
+
+// sensor rays (origin, direction, wavelength) traced toward the object side
+void shim_lens_trace(const float* surfaces, int ns, const float* settings, const float* o, const float* d,
+                     const float* lambda, float* o_out, float* d_out, uint32_t* ok, int n) {
+    const Lens L = make_lens(surfaces, ns, settings);
+    for (int i = 0; i < n; i++) {
+        vec3 a(o[3 * i], o[3 * i + 1], o[3 * i + 2]), b(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        ok[i] = lens_trace(a, b, lambda[i], L);
+        o_out[3 * i] = a.x; o_out[3 * i + 1] = a.y; o_out[3 * i + 2] = a.z;
+        d_out[3 * i] = b.x; d_out[3 * i + 1] = b.y; d_out[3 * i + 2] = b.z;
+    }
+}
+
+// camera rays of pixel (x, y) of a W x H image with the identity camera
+// transform, sample s of RNG stream (seed, index + i)
+void shim_lens_camera_ray(const float* surfaces, int ns, const float* settings, uint32_t W, uint32_t H, uint32_t x,
+                          uint32_t y, uint32_t seed, float* o, float* d, float* w, uint32_t* ok, int n) {
+    const Lens L = make_lens(surfaces, ns, settings);
+    Camera cam{};
+    cam.transform.c0 = vec4(1.0f, 0.0f, 0.0f, 0.0f);
+    cam.transform.c1 = vec4(0.0f, 1.0f, 0.0f, 0.0f);
+    cam.transform.c2 = vec4(0.0f, 0.0f, 1.0f, 0.0f);
+    cam.transform.c3 = vec4(0.0f, 0.0f, 0.0f, 1.0f);
+    for (int i = 0; i < n; i++) {
+        Prng prng(seed, (uint32_t)i);
+        vec3 a, b, c;
+        ok[i] = lens_camera_ray(cam, L, x, y, W, H, prng, a, b, c);
+        o[3 * i] = a.x; o[3 * i + 1] = a.y; o[3 * i + 2] = a.z;
+        d[3 * i] = b.x; d[3 * i + 1] = b.y; d[3 * i + 2] = b.z;
+        w[3 * i] = c.x; w[3 * i + 1] = c.y; w[3 * i + 2] = c.z;
+    }
+}
+
+// CIE 1931 colour matching functions at wavelengths (nm), their linear-sRGB
+// image, and the per-channel normalisation of the spectral weight
+void shim_spectral(const float* nm, int n, float* xyz, float* rgb, float* norm) {
+    for (int i = 0; i < n; i++) {
+        const vec3 v = wavelength_to_xyz(nm[i]);
+        const vec3 c = xyz_to_rgb(v);
+        xyz[3 * i] = v.x; xyz[3 * i + 1] = v.y; xyz[3 * i + 2] = v.z;
+        rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
+    }
+    norm[0] = kCieNormRgbR;
+    norm[1] = kCieNormRgbG;
+    norm[2] = kCieNormRgbB;
+}
+
+void shim_xyz_to_rgb(const float* xyz, float* rgb, int n) {
+    for (int i = 0; i < n; i++) {
+        const vec3 c = xyz_to_rgb(vec3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+        rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
     }
 }
 

@@ -30,7 +30,9 @@ def main():
     dev = api.Device(0)
     out = {}
     kept = []
-    for name, desc, lo, hi in (("cornell", scenes.cornell_box(48, 48), 0.02, 0.98),
+    # BVH8 depth >= 3: with one LDS row, the second stack entry is the first
+    # in the spill rows (a depth-2 tree pushes at its root level only)
+    for name, desc, lo, hi in (("tessellated", scenes.tessellated_cornell_box(48, 48, 2000), 0.02, 0.98),
                                ("spheres", scenes.sphere_field(64, 36, grid=3, level=4), -2.5, 2.5)):
         desc.envmap_resolution = 64
         scene = scenes.build_device_scene(dev, desc)
@@ -51,12 +53,12 @@ def main():
         out[name] = dict(bvh_depth=int(depth), rays=int(rays.shape[0]), worst_rel=worst)
         kept.append((name, desc, scene, orc))
     # a batch keeps its own spill rows for the merged scene; its jobs share
-    # one material library, so the second job is a tessellated Cornell box
-    tess = scenes.tessellated_cornell_box(32, 40, 2000)
-    tess.envmap_resolution = 64
-    tess_orc = binding.OracleScene(tess)
-    tess_orc.build_envmap(threads=parity.ORACLE_THREADS)
-    batch = [kept[0], ("tessellated", tess, scenes.build_device_scene(dev, tess), tess_orc)]
+    # one material library, so the second job is the plain Cornell box
+    plain = scenes.cornell_box(32, 40)
+    plain.envmap_resolution = 64
+    plain_orc = binding.OracleScene(plain)
+    plain_orc.build_envmap(threads=parity.ORACLE_THREADS)
+    batch = [kept[0], ("cornell", plain, scenes.build_device_scene(dev, plain), plain_orc)]
     jobs = [(scene, desc.width, desc.height, 2, 6, 0) for (_, desc, scene, _) in batch]
     b = api.Batch(dev, jobs)
     b.render(queue_paths=4096)

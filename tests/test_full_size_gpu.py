@@ -329,5 +329,8 @@ def test_traversal_spill_rows():
     log(p.stdout[-2000:])
     assert p.returncode == 0, p.stderr[-4000:]
     res = json.loads(p.stdout.strip().splitlines()[-1])
-    for name, v in res.items():
-        assert v["bvh_depth"] >= 3, (name, v)
+    # the scenes traced alone reach the spill rows (depth >= 3); the batch
+    # entries carry only their image error
+    for name in ("tessellated", "spheres"):
+        assert res[name]["bvh_depth"] >= 3, (name, res[name])
+    assert set(res) == {"tessellated", "spheres", "cornell"} and "batch_worst_rel" in res["cornell"]
