@@ -215,6 +215,21 @@ int rm_trace_closest(rm_scene* scene, const float* rays, uint32_t n, float* t, u
                      float* bary);
 int rm_trace_any(rm_scene* scene, const float* rays, uint32_t n, uint32_t* occluded);
 
+/* Allocation-free closest-hit batch.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Results
+ * equal rm_trace_closest's; the difference is that rays, t, inst, prim and
+ * bary are device arrays in the same layouts (rays 16-byte aligned, bary
+ * 8-byte aligned) and the call only enqueues work on `stream` (a
+ * hipStream_t; null: the device's stream) without synchronising.  The
+ * intermediate state lives in a fixed device array of the library (10.5 MB
+ * per GPU, kept until the process ends), so a call takes at most 262144
+ * rays (RM_ERROR_INVALID beyond) and calls that overlap on one GPU race.
+ * Deep traversal stacks use the scene's spill rows, allocated when the
+ * scene was built, as rm_trace_closest does. */
+int rm_trace_closest_noalloc(rm_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* inst,
+                             uint32_t* prim, float* bary, void* stream);
+
 /* ---- renderer ----------------------------------------------------------- */
 
 typedef struct rm_uniforms {

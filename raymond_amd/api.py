@@ -190,6 +190,8 @@ EXPORTS = {
     "rm_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
     "rm_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "rm_trace_closest_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
     "rm_renderer_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(C.c_void_p)]),
     "rm_renderer_destroy": (None, [C.c_void_p]),
@@ -472,6 +474,11 @@ class Scene:
         occ = np.zeros(n, np.uint32)
         _check(LIB.rm_trace_any(self.handle, rays.ctypes.data, n, occ.ctypes.data))
         return occ
+
+    def trace_closest_noalloc(self, d_rays, n, d_t, d_inst, d_prim, d_bary, stream=None):
+        """rm_trace_closest_noalloc on device arrays (addresses as ints): only
+        enqueues the work on `stream` (None: the device's stream)."""
+        _check(LIB.rm_trace_closest_noalloc(self.handle, d_rays, n, d_t, d_inst, d_prim, d_bary, stream))
 
     def close(self):
         if self.handle:

@@ -27,7 +27,7 @@ HOST_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
-KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip", "kernels/rcache.hip"]
+KERNELS = ["kernels/trace.hip", "kernels/misc.hip", "kernels/sort.hip", "kernels/rcache.hip", "kernels/noalloc.hip"]
 HOST = ["host/bvh_build.cpp", "host/jit.cpp", "host/material_builder.cpp", "host/bvh_check.cpp", "host/scene.cpp", "host/renderer.cpp", "host/rcache.cpp", "host/batch.cpp",
         "host/capi.cpp", "host/ply_reader.cpp", "host/exr_writer.cpp", "host/sky.cpp"]
 
@@ -47,7 +47,7 @@ def _run(cmd):
 # define; the other objects are shared with the main build.
 SPILL_LIB = os.path.join(HERE, "libraymond_amd_spill.so")
 SPILL_DEFINES = ["-DRM_LDS_STACK=1"]
-SPILL_SOURCES = ["kernels/trace.hip", "host/scene.cpp", "host/renderer.cpp", "host/batch.cpp",
+SPILL_SOURCES = ["kernels/trace.hip", "kernels/noalloc.hip", "host/scene.cpp", "host/renderer.cpp", "host/batch.cpp",
                  "host/capi.cpp"]
 
 

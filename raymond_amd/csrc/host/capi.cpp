@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../kernels/misc_api.h"
+#include "../kernels/noalloc_api.h"
 #include "../kernels/trace_api.h"
 #include "core.h"
 #include "batch_api.h"
@@ -64,6 +65,7 @@ int rm_device_create(int ordinal, rm_device** out) {
         d->any_blocks_per_cu = rm::trace_blocks_per_cu(true);
         if (d->trace_blocks_per_cu < 1) d->trace_blocks_per_cu = 1;
         if (d->any_blocks_per_cu < 1) d->any_blocks_per_cu = 1;
+        rm::noalloc_init(d);
     } catch (...) {
         delete d;
         throw;
@@ -198,6 +200,21 @@ int rm_trace_closest(rm_scene* scene, const float* rays, uint32_t n, float* t, u
     if (n) RM_NULL_CHECK(rays);
     RM_GUARD_BEGIN
     rm::trace_closest_raw(scene, rays, n, t, inst, prim, bary);
+    RM_GUARD_END
+}
+
+int rm_trace_closest_noalloc(rm_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* inst,
+                             uint32_t* prim, float* bary, void* stream) {
+    RM_NULL_CHECK(scene);
+    if (n) {
+        RM_NULL_CHECK(rays);
+        RM_NULL_CHECK(t);
+        RM_NULL_CHECK(inst);
+        RM_NULL_CHECK(prim);
+        RM_NULL_CHECK(bary);
+    }
+    RM_GUARD_BEGIN
+    rm::trace_closest_noalloc(scene, rays, n, t, inst, prim, bary, static_cast<hipStream_t>(stream));
     RM_GUARD_END
 }
 

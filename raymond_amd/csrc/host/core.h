@@ -100,6 +100,8 @@ struct rm_device {
     hipDeviceProp_t props{};
     int trace_blocks_per_cu = 1;
     int any_blocks_per_cu = 1;
+    // this device's address of the allocation-free entry point's fixed array
+    uint32_t* noalloc_scratch = nullptr;
 };
 
 struct rm_scene {

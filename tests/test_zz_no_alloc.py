@@ -1,0 +1,98 @@
+"""rm_trace_closest_noalloc, the entry point that allocates nothing: its
+hits equal rm_trace_closest's bit for bit on two calls in a row, the second
+with fewer rays (the fixed scratch array keeps the first call's contents),
+and it rejects batches beyond its maximum.  Device arrays come from the HIP
+runtime directly."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from tests import parity
+from tests.test_gpu_parity import edge_rays, random_rays
+
+pytestmark = pytest.mark.gpu
+
+MAX_RAYS = 1 << 18
+
+
+class Hip:
+    """hipMalloc / hipMemcpy / hipFree through the HIP runtime library."""
+
+    def __init__(self):
+        self.lib = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipDeviceSynchronize.argtypes = []
+        self.bufs = []
+
+    def check(self, status, what):
+        assert status == 0, "%s failed: %d" % (what, status)
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.hipMalloc(C.byref(p), max(nbytes, 256)), "hipMalloc")
+        self.bufs.append(p.value)
+        return p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        self.check(self.lib.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return p
+
+    def download(self, p, shape, dtype):
+        out = np.zeros(shape, dtype)
+        self.check(self.lib.hipMemcpy(out.ctypes.data, p, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def sync(self):
+        self.check(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def free(self):
+        for p in self.bufs:
+            self.lib.hipFree(p)
+        self.bufs = []
+
+
+def noalloc_trace(hip, scene, rays):
+    n = rays.shape[0]
+    d_rays = hip.upload(rays.astype(np.float32))
+    d_t, d_inst, d_prim, d_bary = hip.alloc(4 * n), hip.alloc(4 * n), hip.alloc(4 * n), hip.alloc(8 * n)
+    scene.trace_closest_noalloc(d_rays, n, d_t, d_inst, d_prim, d_bary)
+    hip.sync()
+    return (hip.download(d_t, n, np.float32), hip.download(d_inst, n, np.uint32),
+            hip.download(d_prim, n, np.uint32), hip.download(d_bary, (n, 2), np.float32))
+
+
+@pytest.fixture(scope="module")
+def spheres(device):
+    desc = scenes.sphere_field(96, 54, grid=3, level=4)
+    desc.envmap_resolution = 64
+    return desc, scenes.build_device_scene(device, desc)
+
+
+def test_noalloc_matches_rm_trace_closest(spheres):
+    desc, scene = spheres
+    hip = Hip()
+    try:
+        big = np.concatenate([random_rays(150000, -2.5, 2.5, 61), edge_rays(desc, 50000, 62)])
+        small = random_rays(40000, -2.5, 2.5, 63, tmin=1e-3, tmax=2.0)
+        for rays in (big, small):  # the second call is smaller: stale scratch must not leak
+            got = noalloc_trace(hip, scene, rays)
+            parity.assert_hits_equal(got, scene.trace_closest(rays))
+            assert np.mean(got[0] >= 0) > 0.2
+        # at the limit, and beyond it: rejected before anything is enqueued
+        full = random_rays(MAX_RAYS, -2.5, 2.5, 64)
+        parity.assert_hits_equal(noalloc_trace(hip, scene, full), scene.trace_closest(full))
+        d = hip.alloc(16)
+        with pytest.raises(api.RmError, match="at most 262144"):
+            scene.trace_closest_noalloc(d, MAX_RAYS + 1, d, d, d, d)
+        scene.trace_closest_noalloc(None, 0, None, None, None, None)  # empty batch: nothing to do
+    finally:
+        hip.sync()
+        hip.free()
