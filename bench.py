@@ -13,6 +13,13 @@ radiance cache, RMSE against a 16384-spp FP64 reference at power-of-two spp,
 and the cached tracer's time to the plain tracer's 64-spp RMSE over the plain
 time (`time_ratio`).
 
+`roofline` is trace_closest's: `achieved` is its algorithmic bytes (28 B ray +
+20 B hit + 80 B per node visit + 48 B per triangle test) over its summed HIP-event
+time; `traffic` is its HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE
+counter runs recorded in profiles/trace_closest_c2_traffic.json, and
+`hbm_measured` the same bytes over this run's time, the rate the HBM actually
+delivers.
+
 `--workload graphs` runs the shader-graph workload instead: the Cornell-box geometry
 with 32 random node-graph materials, depth 8, shading sorted by material.
 
@@ -419,6 +426,9 @@ def main():
             tr = json.load(f)
         # one closest-hit launch per bounce-loop iteration
         traffic = tr["traffic_bytes_per_ray"] * closest_total / max(totals["iters"], 1)
+    # the measured HBM rate of the same launches: counter bytes over their time
+    hbm_measured = None if traffic is None or totals["ms_closest"] <= 0 else \
+        traffic * max(totals["iters"], 1) / (totals["ms_closest"] * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -475,6 +485,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_source": None if traffic is None else "profiles/trace_closest_c2_traffic.json",
+                     "hbm_measured": None if hbm_measured is None else round(hbm_measured, 1),
+                     "hbm_measured_frac": None if hbm_measured is None else round(hbm_measured / HBM_PEAK_GBS, 4),
                      "bytes_per_ray": round(bytes_per_ray, 1),
                      "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2)},
         "cpu_baseline": cpu,
