@@ -84,13 +84,14 @@ def bench_batch(args):
           file=sys.stderr, flush=True)
     t1 = time.time()
     cfg, tried, hit = batch.autotune(rr_depth=3)
-    print("autotune %.1fs (cache %s): queue %d, sort %d; tried %s" % (time.time() - t1, hit, cfg.queue_paths,
-                                                                      cfg.material_sort, tried),
-          file=sys.stderr, flush=True)
+    print("autotune %.1fs (cache %s): queue %d, sort %d, trace grid %d, sort shift %d; tried %s"
+          % (time.time() - t1, hit, cfg.queue_paths, cfg.material_sort, cfg.trace_blocks_per_cu, cfg.sort_key_shift,
+             tried), file=sys.stderr, flush=True)
     # the bounce loop replays captured hipGraph chunks (same speed as direct
     # launches at these queue depths: 2999 against 2998 Mrays/s)
     st = batch.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort, rr_depth=3, timing=0,
-                      use_graph=1, order=api.BATCH_SHORTEST_FIRST)
+                      use_graph=1, order=api.BATCH_SHORTEST_FIRST, trace_blocks_per_cu=cfg.trace_blocks_per_cu,
+                      sort_key_shift=cfg.sort_key_shift)
     secs = st.ms_total / 1e3
     mrays = st.total_rays / secs / 1e6
     times = batch.job_times()
@@ -111,9 +112,11 @@ def bench_batch(args):
                                (len(specs), tris),
                    "jobs": len(specs), "triangles": tris, "samples": paths,
                    "queue_paths": cfg.queue_paths, "material_sort": bool(cfg.material_sort), "hipgraph": True,
+                   "trace_blocks_per_cu": cfg.trace_blocks_per_cu, "sort_key_shift": cfg.sort_key_shift,
                    "order": "shortest job first",
                    "autotune": {"cache_hit": hit, "candidates": [
-                       {"queue_paths": q, "material_sort": m, "mrays_per_s": round(r, 1)} for q, m, r in tried]}},
+                       {"queue_paths": q, "material_sort": m, "trace_blocks_per_cu": tb, "sort_key_shift": sh,
+                        "mrays_per_s": round(r, 1)} for q, m, tb, sh, r in tried]}},
         "seconds": round(secs, 3),
         "samples_per_s": round(st.samples / secs, 1),
         "bounce_loops": st.loops,

@@ -182,6 +182,15 @@ void rm_scene_destroy(rm_scene* scene);
 /* Builds the BVH8 over all instances (replaces the Metal primitive and
  * instance acceleration structures, ShapeBuilder.swift:7-94, EntityBuilder.swift:146-165). */
 int rm_scene_build_accel(rm_scene* scene, int host_threads);
+/* Builds the two-level form instead: one BVH8 per shape (a distinct vertex
+ * and face range) over its object-space triangles, and one over the
+ * instances' world boxes whose leaves map the ray into the instance's object
+ * space (the reference's primitive and instance acceleration structures,
+ * ShapeBuilder.swift:7-94, EntityBuilder.swift:146-165).  A shape placed by
+ * many instances is stored once.  Closest and any hit then traverse both
+ * levels; hit distances are the same ray parameter as in the flat form, up
+ * to the rounding of the object-space map. */
+int rm_scene_build_accel_two_level(rm_scene* scene, int host_threads);
 /* Compiles generated material source (shade_surface / shade_light switch
  * tables) together with the shading kernels via hiprtc.  `log` receives the
  * compiler log (may be NULL). (MaterialBuilder.swift:253-325) */
@@ -198,7 +207,9 @@ typedef struct rm_scene_stats {
     double bvh_build_seconds;
     double jit_seconds;
     double envmap_seconds;
-    uint64_t device_bytes;
+    uint64_t device_bytes;      /* the scene's device buffers: geometry, trees, lights, envmap, textures, spill rows */
+    uint64_t stored_triangles;  /* triangles in the acceleration structure (two-level: each shape once) */
+    uint32_t two_level;         /* built by rm_scene_build_accel_two_level */
 } rm_scene_stats;
 int rm_scene_get_stats(rm_scene* scene, rm_scene_stats* out);
 
@@ -243,6 +254,8 @@ typedef struct rm_uniforms {
     uint32_t traversal_stats; /* count BVH nodes / triangles visited (slower) */
     uint32_t material_sort;   /* group each iteration's hits by material before shading */
     uint32_t use_graph;       /* replay the bounce loop from a captured hipGraph (ignored with timing) */
+    uint32_t trace_blocks_per_cu; /* persistent trace workgroups per CU; 0 or more than fit = the most that fit */
+    uint32_t sort_key_shift;  /* material-sort granularity: materials m >> shift share a bin (0: one per material) */
 } rm_uniforms;
 
 typedef struct rm_render_stats {
@@ -411,6 +424,8 @@ typedef struct rm_batch_config {
                               * count, so small jobs complete early and fewer jobs share the
                               * caches at a time; RM_BATCH_SAMPLE_MAJOR: every job progresses
                               * from the start */
+    uint32_t trace_blocks_per_cu; /* persistent trace workgroups per CU (0: the most that fit) */
+    uint32_t sort_key_shift; /* material-sort granularity (rm_uniforms.sort_key_shift) */
 } rm_batch_config;
 
 enum { RM_BATCH_SHORTEST_FIRST = 0, RM_BATCH_SAMPLE_MAJOR = 1 };
@@ -425,7 +440,7 @@ typedef struct rm_batch_stats {
 } rm_batch_stats;
 
 typedef struct rm_batch_tune_entry {
-    uint32_t queue_paths, material_sort;
+    uint32_t queue_paths, material_sort, trace_blocks_per_cu, sort_key_shift;
     double mrays_per_s;
 } rm_batch_tune_entry;
 
@@ -437,7 +452,9 @@ int rm_batch_job_times(rm_batch* b, double* ms, uint32_t n);
 /* Accumulated RGBA sums of one job (width*height*4 floats; divide by samples). */
 int rm_batch_read_image(rm_batch* b, uint32_t job, float* rgba, size_t count);
 /* Picks queue depth and material sort for this batch by timing a prefix of
- * it (the first samples of every job) under each candidate, or takes the
+ * it (the first samples of every job) under each candidate, then, from the
+ * best of those, the trace grid (workgroups per CU) and the sort granularity
+ * (materials per bin); or takes the
  * choice recorded for the same device and batch signature in the tuner cache
  * ($RM_TUNE_CACHE, else ~/.cache/raymond_amd/batch_tune.txt).  `best` enters
  * with the other settings, which are kept.  entries/count report the

@@ -15,6 +15,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include "rm.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -43,6 +44,18 @@ void rmh_free(void* p);
  * (A test hook for the builder behind rm_scene_build_accel.) */
 int rmh_bvh8_check(const float* tris, uint32_t n, int threads, uint32_t* out_nodes, uint32_t* out_depth, char* err,
                    size_t err_cap);
+/* Builds the two-level form of rm_scene_build_accel_two_level on the host
+ * from scene arrays (vertices 3 floats each, faces 3 indices relative to the
+ * instance's vertex_offset, instance descriptions) and traces n rays (8
+ * floats: origin, tmin, direction, tmax) through it on the CPU with the
+ * device traversal code: closest hits (any = 0) as rm_trace_closest writes
+ * them, or the first hit found (any = 1).  out_stored receives the
+ * triangles stored (each shape once).  Returns 0, or -1 and a message.  (A
+ * test hook for the traversal behind the two-level form.) */
+int rmh_two_level_trace(const float* vertices, uint32_t num_vertices, const uint32_t* indices, uint32_t num_faces,
+                        const rm_instance_desc* instances, uint32_t num_instances, const float* rays, uint32_t n,
+                        int any, int threads, float* t, uint32_t* inst, uint32_t* prim, float* bary,
+                        uint64_t* out_stored, char* err, size_t err_cap);
 /* Compiles generated material source for gfx950 with hiprtc, without a device,
  * into the code-object cache that rm_scene_compile_materials reads (the
  * directory in $RM_JIT_CACHE, else ~/.cache/raymond_amd).  Same source and

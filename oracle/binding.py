@@ -56,6 +56,7 @@ def lib():
         L.or_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_int, C.c_int]
         L.or_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_int]
+        L.or_scene_set_two_level.argtypes = [C.c_void_p, C.c_int]
         L.or_render.argtypes = [C.c_void_p, C.POINTER(RenderArgs), C.c_void_p, C.POINTER(RenderResult)]
         L.or_render_f64.argtypes = [C.c_void_p, C.POINTER(RenderArgs), C.c_void_p, C.POINTER(RenderResult)]
         _LIB = L
@@ -100,6 +101,11 @@ class OracleScene:
         self.handle = L.or_scene_create(C.addressof(sd), so.encode(), err, 512)
         if not self.handle:
             raise RuntimeError("oracle scene: " + err.value.decode())
+
+    def set_two_level(self, on=True):
+        """Hits through the two-level form (every instance in its object space)."""
+        if lib().or_scene_set_two_level(self.handle, int(bool(on))) != 0:
+            raise RuntimeError("oracle scene: singular instance transform")
 
     def build_envmap(self, threads=0):
         lib().or_build_envmap(self.handle, threads)

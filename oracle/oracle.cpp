@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include "../raymond_amd/csrc/device/bvh2l.h"
 #include "../raymond_amd/csrc/device/isect.h"
 #include "../raymond_amd/csrc/device/material_prelude.h"
 #include "../raymond_amd/csrc/device/camera.h"
@@ -82,6 +83,10 @@ struct or_scene {
     std::vector<WTri> tris;
     std::vector<BNode> bvh;
     std::vector<uint32_t> bvh_idx;
+    std::vector<uint32_t> inst_faces;   // face count of each instance
+    // two-level scenes: world-to-object rows per instance (or_scene_set_two_level)
+    bool two_level = false;
+    std::vector<rm::TlasInstance> tlas;
     void* so = nullptr;
     MatTable mat;
 };
@@ -217,8 +222,37 @@ bool test_tri(const WTri& t, vec3 o, const rm::RayShear& sh, float tmin, float t
     return true;
 }
 
+// Two-level scenes (the instance acceleration structure of the reference,
+// EntityBuilder.swift:146-165): every instance, the world ray mapped into its
+// object space by its world-to-object rows, every triangle of its shape in
+// object space.  Exhaustive, so it checks the product's culling too.
+Hit trace_two_level(const or_scene& s, vec3 wo, vec3 wd, float tmin, float tmax, bool any) {
+    Hit best;
+    for (uint32_t k = 0; k < (uint32_t)s.instances.size(); k++) {
+        const rm::TlasInstance& ti = s.tlas[k];
+        const vec3 o = rm::tlas_point(ti, wo), d = rm::tlas_vector(ti, wd);
+        const rm::RayShear sh = rm::make_shear(d);
+        const rm::PerInstance& pi = s.instances[k];
+        for (uint32_t f = 0; f < s.inst_faces[k]; f++) {
+            const uint32_t* ix = &s.indices[3 * ((size_t)pi.face_offset + f)];
+            WTri t;
+            const float* p0 = &s.vertices[3 * ((size_t)pi.vertex_offset + ix[0])];
+            const float* p1 = &s.vertices[3 * ((size_t)pi.vertex_offset + ix[1])];
+            const float* p2 = &s.vertices[3 * ((size_t)pi.vertex_offset + ix[2])];
+            t.v0 = vec3(p0[0], p0[1], p0[2]);
+            t.v1 = vec3(p1[0], p1[1], p1[2]);
+            t.v2 = vec3(p2[0], p2[1], p2[2]);
+            t.inst = k;
+            t.prim = f;
+            if (test_tri(t, o, sh, tmin, tmax, best) && any) return best;
+        }
+    }
+    return best;
+}
+
 // closest hit (or any hit) with t in (tmin, tmax)
 Hit trace(const or_scene& s, vec3 o, vec3 d, float tmin, float tmax, bool any, bool brute) {
+    if (s.two_level) return trace_two_level(s, o, d, tmin, tmax, any);
     Hit best;
     const rm::RayShear sh = rm::make_shear(d);
     if (brute) {
@@ -558,6 +592,7 @@ or_scene* or_scene_create(const rm_scene_desc* d, const char* material_so, char*
     uint32_t light_face_offset = 0;
     for (uint32_t k = 0; k < d->num_instances; k++) {
         const rm_instance_desc& id = d->instances[k];
+        s->inst_faces.push_back(id.face_count);
         rm::PerInstance pi{};
         pi.vertex_offset = id.vertex_offset;
         pi.face_offset = id.face_offset;
@@ -892,6 +927,14 @@ int or_render(or_scene* s, const or_render_args* A, float* image, or_render_resu
 int or_render_f64(or_scene* s, const or_render_args* A, double* image, or_render_result* res) {
     if (!image) return -1;
     return render_impl(s, A, nullptr, image, res);
+}
+
+int or_scene_set_two_level(or_scene* s, int on) {
+    s->two_level = on != 0;
+    s->tlas.assign(s->instances.size(), rm::TlasInstance{});
+    for (size_t k = 0; k < s->instances.size(); k++)
+        if (!rm::tlas_inverse(s->instances[k].point_transform, s->tlas[k].m)) return -1;
+    return 0;
 }
 
 }  // extern "C"

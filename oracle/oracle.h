@@ -52,6 +52,11 @@ int or_read_light_tables(or_scene* s, float* light_faces, float* emissive_areas)
 int or_trace_closest(or_scene* s, const float* rays, uint32_t n, float* t, uint32_t* inst, uint32_t* prim, float* bary,
                      int brute_force, int threads);
 int or_trace_any(or_scene* s, const float* rays, uint32_t n, uint32_t* occluded, int brute_force, int threads);
+/* Two-level hits (the product's rm_scene_build_accel_two_level): every
+ * instance, the ray mapped into its object space, every triangle of its
+ * shape.  on = 0 returns to the flat world-space triangles.  -1 for a
+ * singular instance transform. */
+int or_scene_set_two_level(or_scene* s, int on);
 
 typedef struct or_render_args {
     uint32_t width, height;

@@ -111,14 +111,14 @@ class SceneDesc(C.Structure):
 class SceneStats(C.Structure):
     _fields_ = [("triangles", C.c_uint64), ("bvh_nodes", C.c_uint64), ("bvh_depth", C.c_uint32),
                 ("bvh_build_seconds", C.c_double), ("jit_seconds", C.c_double), ("envmap_seconds", C.c_double),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("stored_triangles", C.c_uint64), ("two_level", C.c_uint32)]
 
 
 class Uniforms(C.Structure):
     _fields_ = [("sampling_mode", C.c_uint32), ("accumulate", C.c_uint32), ("exposure", C.c_float),
                 ("tonemapping", C.c_uint32), ("rr_depth", C.c_uint32), ("accumulate_f64", C.c_uint32),
                 ("timing", C.c_uint32), ("traversal_stats", C.c_uint32), ("material_sort", C.c_uint32),
-                ("use_graph", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("sort_key_shift", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -151,7 +151,8 @@ BATCH_SHORTEST_FIRST, BATCH_SAMPLE_MAJOR = 0, 1
 
 class BatchConfig(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("rr_depth", C.c_uint32),
-                ("timing", C.c_uint32), ("use_graph", C.c_uint32), ("order", C.c_uint32)]
+                ("timing", C.c_uint32), ("use_graph", C.c_uint32), ("order", C.c_uint32),
+                ("trace_blocks_per_cu", C.c_uint32), ("sort_key_shift", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -163,7 +164,8 @@ class BatchStats(C.Structure):
 
 
 class BatchTuneEntry(C.Structure):
-    _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("mrays_per_s", C.c_double)]
+    _fields_ = [("queue_paths", C.c_uint32), ("material_sort", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("sort_key_shift", C.c_uint32), ("mrays_per_s", C.c_double)]
 
 
 CACHE_FEATURES = 12
@@ -180,6 +182,7 @@ EXPORTS = {
     "rm_scene_create": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "rm_scene_destroy": (None, [C.c_void_p]),
     "rm_scene_build_accel": (C.c_int, [C.c_void_p, C.c_int]),
+    "rm_scene_build_accel_two_level": (C.c_int, [C.c_void_p, C.c_int]),
     "rm_scene_compile_materials": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p,
                                              C.c_size_t]),
     "rm_scene_build_envmap": (C.c_int, [C.c_void_p]),
@@ -235,6 +238,9 @@ EXPORTS = {
     "rmh_jit_precompile": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "rmh_bvh8_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.c_char_p, C.c_size_t]),
+    "rmh_two_level_trace": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]),
 }
 
 
@@ -388,6 +394,29 @@ def bvh8_check(tris, threads=0):
     return nodes.value, depth.value
 
 
+def two_level_trace(vertices, faces, instances, rays, any_hit=False, threads=0):
+    """Host build of the two-level form and a CPU trace through it
+    (rmh_two_level_trace).  instances: InstanceDesc array (vertex_offset,
+    face_offset, face_count and transform used).  Returns (t, inst, prim,
+    bary, stored_triangles)."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = r.shape[0]
+    t = np.zeros(n, np.float32)
+    inst = np.zeros(n, np.uint32)
+    prim = np.zeros(n, np.uint32)
+    bary = np.zeros((n, 2), np.float32)
+    stored = C.c_uint64()
+    err = C.create_string_buffer(256)
+    status = LIB.rmh_two_level_trace(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0], C.addressof(instances),
+                                     len(instances), r.ctypes.data, n, int(bool(any_hit)), threads, t.ctypes.data,
+                                     inst.ctypes.data, prim.ctypes.data, bary.ctypes.data, C.byref(stored), err, 256)
+    if status != 0:
+        raise RmError(err.value.decode())
+    return t, inst, prim, bary, stored.value
+
+
 class Device:
     def __init__(self, ordinal=0):
         self.handle = C.c_void_p()
@@ -423,8 +452,11 @@ class Scene:
         sd = desc.to_c()
         _check(LIB.rm_scene_create(device.handle, C.byref(sd), C.byref(self.handle)))
 
-    def build_accel(self, threads=0):
-        _check(LIB.rm_scene_build_accel(self.handle, threads))
+    def build_accel(self, threads=0, two_level=False):
+        if two_level:
+            _check(LIB.rm_scene_build_accel_two_level(self.handle, threads))
+        else:
+            _check(LIB.rm_scene_build_accel(self.handle, threads))
 
     def compile_materials(self, source, options=""):
         log = C.create_string_buffer(1 << 16)
@@ -641,8 +673,10 @@ class Batch:
         self.handle = C.c_void_p()
         _check(LIB.rm_batch_create(device.handle, arr, len(jobs), C.byref(self.handle)))
 
-    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0, use_graph=0, order=BATCH_SHORTEST_FIRST):
-        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing, use_graph, order)
+    def render(self, queue_paths=0, material_sort=0, rr_depth=0, timing=0, use_graph=0, order=BATCH_SHORTEST_FIRST,
+               trace_blocks_per_cu=0, sort_key_shift=0):
+        cfg = BatchConfig(queue_paths, material_sort, rr_depth, timing, use_graph, order, trace_blocks_per_cu,
+                          sort_key_shift)
         st = BatchStats()
         _check(LIB.rm_batch_render(self.handle, C.byref(cfg), C.byref(st)))
         return st
@@ -664,7 +698,8 @@ class Batch:
         count = C.c_uint32(0)
         hit = C.c_int(0)
         _check(LIB.rm_batch_autotune(self.handle, C.byref(cfg), entries, 16, C.byref(count), C.byref(hit)))
-        tried = [(e.queue_paths, e.material_sort, e.mrays_per_s) for e in entries[:count.value]]
+        tried = [(e.queue_paths, e.material_sort, e.trace_blocks_per_cu, e.sort_key_shift, e.mrays_per_s)
+                 for e in entries[:count.value]]
         return cfg, tried, bool(hit.value)
 
     def close(self):

@@ -67,6 +67,7 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
         RM_REQUIRE(d.scene->accel_built && d.scene->materials_built && d.scene->envmap_built,
                    "job scenes must have their acceleration structure, materials and envmap built");
         RM_REQUIRE(d.scene->material_source == src, "the jobs of a batch must share one material library");
+        RM_REQUIRE(!d.scene->two_level, "batches take scenes with the flat acceleration structure");
         RM_REQUIRE(d.width > 0 && d.height > 0 && d.samples > 0, "job size and samples must be positive");
         RM_REQUIRE(d.max_depth >= 1 && d.max_depth <= 255, "job max_depth out of range");
         pixels += (uint64_t)d.width * d.height;
@@ -218,6 +219,8 @@ void batch_render(rm_batch* b, const rm_batch_config* cfg, rm_batch_stats* stats
     r->u.material_sort = c.material_sort;
     r->u.timing = c.timing;
     r->u.use_graph = c.use_graph;
+    r->u.trace_blocks_per_cu = c.trace_blocks_per_cu;
+    r->u.sort_key_shift = c.sort_key_shift;
     r->u.traversal_stats = 0;
     renderer_reset(r);
     const uint32_t n = (uint32_t)b->jobs.size();
@@ -312,7 +315,8 @@ void batch_read_image(rm_batch* b, uint32_t job, float* rgba, size_t count) {
 
 namespace {
 
-// Tuner cache: one line per (device, batch signature): queue_paths material_sort.
+// Tuner cache: one line per (device, batch signature): queue_paths material_sort
+// trace_blocks_per_cu sort_key_shift.
 std::string tune_cache_path() {
     if (const char* e = getenv("RM_TUNE_CACHE")) return e;
     if (const char* h = getenv("HOME")) return std::string(h) + "/.cache/raymond_amd/batch_tune.txt";
@@ -343,10 +347,13 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
         while (std::getline(f, line)) {
             std::istringstream is(line);
             std::string k;
-            uint32_t q = 0, srt = 0;
+            uint32_t q = 0, srt = 0, tb = 0, sh = 0;
             if (is >> k >> q >> srt && k == key && q > 0) {
+                is >> tb >> sh;  // absent in lines of older versions: defaults
                 best->queue_paths = q;
                 best->material_sort = srt;
+                best->trace_blocks_per_cu = tb;
+                best->sort_key_shift = sh;
                 if (cache_hit) *cache_hit = 1;
                 return;
             }
@@ -364,27 +371,55 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
         prefix_paths += (uint64_t)ji.width * ji.height;
         prefix++;
     }
-    const uint32_t queues[4] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     uint32_t used = 0;
     double best_rate = -1.0;
-    rm_batch_config base = *best;
+    const rm_batch_config base = *best;
+    auto trial = [&](rm_batch_config c) {
+        c.timing = 0;
+        c.order = RM_BATCH_SAMPLE_MAJOR;  // the calibration prefix mixes all jobs
+        rm_batch_stats s{};
+        batch_render(b, &c, &s, prefix);
+        const double rate = s.total_rays / (s.ms_total * 1e-3) / 1e6;
+        if (entries && used < capacity)
+            entries[used] = rm_batch_tune_entry{c.queue_paths, c.material_sort, c.trace_blocks_per_cu, c.sort_key_shift,
+                                                rate};
+        used++;
+        if (rate > best_rate) {
+            best_rate = rate;
+            *best = c;
+        }
+    };
+    // stage 1: queue depth x material sort, at the default grid and bins
+    const uint32_t queues[4] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     for (uint32_t qi = 0; qi < 4; qi++)
         for (uint32_t srt = 0; srt < 2; srt++) {
             rm_batch_config c = base;
             c.queue_paths = queues[qi];
             c.material_sort = srt;
-            c.timing = 0;
-            c.order = RM_BATCH_SAMPLE_MAJOR;  // the calibration prefix mixes all jobs
-            rm_batch_stats s{};
-            batch_render(b, &c, &s, prefix);
-            const double rate = s.total_rays / (s.ms_total * 1e-3) / 1e6;
-            if (entries && used < capacity) entries[used] = rm_batch_tune_entry{c.queue_paths, c.material_sort, rate};
-            used++;
-            if (rate > best_rate) {
-                best_rate = rate;
-                *best = c;
-            }
+            c.trace_blocks_per_cu = 0;
+            c.sort_key_shift = 0;
+            trial(c);
         }
+    // stage 2, from the best of stage 1: fewer persistent trace workgroups
+    // per CU (less contention for the L2 among the waves of a CU), and
+    // coarser sort bins (cheaper keys and scatter, less coherent shading)
+    const rm_batch_config stage1 = *best;
+    const int most = std::max(b->dev->trace_blocks_per_cu, b->dev->any_blocks_per_cu);
+    std::vector<int> grids;
+    for (int tb : {most - 1, (most + 1) / 2})
+        if (tb >= 1 && tb < most && std::find(grids.begin(), grids.end(), tb) == grids.end()) grids.push_back(tb);
+    for (int tb : grids) {
+        rm_batch_config c = stage1;
+        c.trace_blocks_per_cu = (uint32_t)tb;
+        trial(c);
+    }
+    if (stage1.material_sort) {
+        for (uint32_t sh : {1u, 3u}) {
+            rm_batch_config c = *best;
+            c.sort_key_shift = sh;
+            trial(c);
+        }
+    }
     if (count) *count = std::min(used, capacity);
     best->timing = base.timing;
     best->order = base.order;
@@ -400,7 +435,9 @@ void batch_autotune(rm_batch* b, rm_batch_config* best, rm_batch_tune_entry* ent
         mkdir(dir.c_str(), 0755);
     }
     std::ofstream f(path, std::ios::app);
-    if (f) f << key << " " << best->queue_paths << " " << best->material_sort << "\n";
+    if (f)
+        f << key << " " << best->queue_paths << " " << best->material_sort << " " << best->trace_blocks_per_cu << " "
+          << best->sort_key_shift << "\n";
 }
 
 }  // namespace rm

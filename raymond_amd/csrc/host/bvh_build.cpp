@@ -45,7 +45,6 @@ struct Node2 {
 };
 
 constexpr int kBins = 32;
-constexpr int kMaxLeaf = 3;
 
 struct Builder2 {
     const std::vector<Aabb>& tb;
@@ -54,6 +53,7 @@ struct Builder2 {
     std::vector<Node2> nodes;
     std::atomic<int> next{0};
     std::atomic<int> threads_left{0};
+    uint32_t max_leaf = 3;  // primitives per leaf
 
     Builder2(const std::vector<Aabb>& tb_, const std::vector<std::array<float, 3>>& c, std::vector<uint32_t>& i)
         : tb(tb_), cent(c), idx(i) {}
@@ -118,7 +118,7 @@ struct Builder2 {
                 }
             }
         }
-        if (n <= (uint32_t)kMaxLeaf && (best_axis < 0 || (double)n <= best_cost)) {
+        if (n <= max_leaf && (best_axis < 0 || (double)n <= best_cost)) {
             make_leaf(ni, begin, end);
             return;
         }
@@ -216,6 +216,7 @@ struct Collapse {
     std::vector<float> cost;
     std::vector<uint8_t> how;
     std::vector<uint32_t> first, count;  // primitive range of every BVH2 node
+    uint32_t max_leaf = 3;               // primitives per leaf slot
 
     void run(const std::vector<Node2>& N2, int n2) {
         cost.assign((size_t)n2 * 8, 0.0f);
@@ -256,7 +257,7 @@ struct Collapse {
                 }
             }
             const double internal = area * kCostNode + dist[8];
-            const double leaf = count[n] <= (uint32_t)kMaxLeaf ? area * kCostTri * count[n] : INFINITY;
+            const double leaf = count[n] <= max_leaf ? area * kCostTri * count[n] : INFINITY;
             c[0] = (float)std::min(internal, leaf);
             h[0] = leaf <= internal ? 0 : 1;
             for (int i = 2; i <= 8; i++) {
@@ -286,7 +287,7 @@ struct Collapse {
 
 }  // namespace
 
-Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error) {
+Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error, int max_leaf) {
     const auto t0 = std::chrono::steady_clock::now();
     Bvh8Build res;
     const size_t n = tris.size();
@@ -316,6 +317,7 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
     for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
 
     Builder2 b2(tb, cent, idx);
+    b2.max_leaf = (uint32_t)std::max(1, std::min(max_leaf, 3));
     b2.nodes.resize(2 * n + 1);
     int hw = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
     b2.threads_left = std::max(0, std::min(hw, 64) - 1);
@@ -327,6 +329,7 @@ Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string
     // Collapse: each BVH8 node lists up to 8 BVH2 subtrees as children, as
     // the cost table distributes them.
     Collapse col;
+    col.max_leaf = b2.max_leaf;
     col.run(N2, res.bvh2_nodes);
     struct Pending {
         int n2;      // BVH2 node this BVH8 node expands

@@ -22,8 +22,10 @@ struct Bvh8Build {
     double build_seconds = 0;
 };
 
-// Builds a compressed BVH8 over world-space triangles.  Leaves hold at most
-// 3 triangles; `threads` <= 0 uses all hardware threads.
-Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error);
+// Builds a compressed BVH8 over triangles.  Leaves hold at most `max_leaf`
+// (1..3) triangles; `threads` <= 0 uses all hardware threads.  A primitive
+// that is a box (an instance of a two-level tree) is passed as the
+// degenerate triangle (lo, hi, hi), whose bounds are the box.
+Bvh8Build build_bvh8(const std::vector<BuildTri>& tris, int threads, std::string* error, int max_leaf = 3);
 
 }  // namespace rm

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "scene_api.h"
 #include "../../../include/rm_host.h"
 #include "bvh_build.h"
 
@@ -129,4 +130,55 @@ extern "C" int rmh_bvh8_check(const float* tris, uint32_t n, int threads, uint32
     if (out_nodes) *out_nodes = (uint32_t)b.nodes.size();
     if (out_depth) *out_depth = (uint32_t)c.max_depth;
     return 0;
+}
+
+namespace {
+struct HostStack2l {
+    rm::Entry2l e[rm::kMaxStack2l];
+    void store(int sp, rm::Entry2l v) { e[sp] = v; }
+    rm::Entry2l load(int sp) const { return e[sp]; }
+};
+}  // namespace
+
+extern "C" int rmh_two_level_trace(const float* vertices, uint32_t num_vertices, const uint32_t* indices,
+                                   uint32_t num_faces, const rm_instance_desc* instances, uint32_t num_instances,
+                                   const float* rays, uint32_t n, int any, int threads, float* t, uint32_t* inst,
+                                   uint32_t* prim, float* bary, uint64_t* out_stored, char* err, size_t err_cap) {
+    try {
+        for (uint32_t k = 0; k < num_instances; k++) {
+            const rm_instance_desc& id = instances[k];
+            RM_REQUIRE((uint64_t)id.face_offset + id.face_count <= num_faces, "instance faces out of range");
+            for (uint32_t f = 0; f < id.face_count; f++)
+                for (int j = 0; j < 3; j++)
+                    RM_REQUIRE((uint64_t)id.vertex_offset + indices[3 * ((size_t)id.face_offset + f) + j] < num_vertices,
+                               "instance vertex index out of range");
+        }
+        const rm::TwoLevelBuild b = rm::build_two_level(std::vector<float>(vertices, vertices + 3 * (size_t)num_vertices),
+                                                        std::vector<uint32_t>(indices, indices + 3 * (size_t)num_faces),
+                                                        std::vector<rm_instance_desc>(instances, instances + num_instances),
+                                                        threads);
+        for (uint32_t i = 0; i < n; i++) {
+            const float* r = rays + 8 * (size_t)i;
+            HostStack2l stk;
+            rm::Hit2l h;
+            const rm::vec3 o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+            if (any)
+                rm::traverse2l<true>(b.nodes.data(), b.tris.data(), b.recs.data(), b.root, o, d, r[3], r[7], stk, h);
+            else
+                rm::traverse2l<false>(b.nodes.data(), b.tris.data(), b.recs.data(), b.root, o, d, r[3], r[7], stk, h);
+            t[i] = h.found ? h.t : -1.0f;
+            inst[i] = h.inst;
+            prim[i] = h.prim;
+            bary[2 * (size_t)i] = h.u / h.det;
+            bary[2 * (size_t)i + 1] = h.v / h.det;
+        }
+        if (out_stored) *out_stored = b.stored;
+        return 0;
+    } catch (const rm::HipError& e) {
+        if (err && err_cap) snprintf(err, err_cap, "%s", e.msg.c_str());
+        return -1;
+    } catch (const std::exception& e) {
+        if (err && err_cap) snprintf(err, err_cap, "%s", e.what());
+        return -1;
+    }
 }

@@ -118,6 +118,13 @@ int rm_scene_build_accel(rm_scene* scene, int host_threads) {
     RM_GUARD_END
 }
 
+int rm_scene_build_accel_two_level(rm_scene* scene, int host_threads) {
+    RM_NULL_CHECK(scene);
+    RM_GUARD_BEGIN
+    rm::scene_build_accel_two_level(scene, host_threads);
+    RM_GUARD_END
+}
+
 int rm_scene_compile_materials(rm_scene* scene, const char* source, size_t length, const char* options, char* log,
                                size_t log_capacity) {
     RM_NULL_CHECK(scene);
@@ -159,6 +166,15 @@ int rm_scene_get_stats(rm_scene* scene, rm_scene_stats* out) {
     RM_NULL_CHECK(scene);
     RM_NULL_CHECK(out);
     *out = scene->stats;
+    const rm::DevBuf* bufs[] = {&scene->d_vertices, &scene->d_normals, &scene->d_texcoords, &scene->d_indices,
+                                &scene->d_materials, &scene->d_instances, &scene->d_area, &scene->d_point,
+                                &scene->d_sun, &scene->d_spot, &scene->d_shape, &scene->d_light_faces,
+                                &scene->d_nodes, &scene->d_tris, &scene->d_tlas, &scene->d_env_pdfs,
+                                &scene->d_env_mip, &scene->d_spill, &scene->d_tex_table};
+    uint64_t bytes = 0;
+    for (const rm::DevBuf* b : bufs) bytes += b->bytes;
+    for (const auto& t : scene->d_tex) bytes += t ? t->bytes : 0;
+    out->device_bytes = bytes;
     return RM_OK;
 }
 

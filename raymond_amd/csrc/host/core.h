@@ -124,6 +124,10 @@ struct rm_scene {
     rm::DevBuf d_vertices, d_normals, d_texcoords, d_indices, d_materials, d_instances;
     rm::DevBuf d_area, d_point, d_sun, d_spot, d_shape, d_light_faces;
     rm::DevBuf d_nodes, d_tris;
+    // two-level form: instance records of the top-level tree and its root
+    rm::DevBuf d_tlas;
+    uint32_t root2l = 0;
+    bool two_level = false;
     rm::DevBuf d_env_pdfs, d_env_mip;
     rm::DevBuf d_spill;
     size_t spill_threads = 0;
@@ -181,4 +185,17 @@ struct rm_renderer {
     rm::CacheView cv{};
     rm::vec3 world_lo, world_hi;
     rm::BatchRun* batch = nullptr;     // set while a batch renders through this renderer
+    // Completion polls of the bounce loop, one chunk behind the GPU: two
+    // pinned slots (the next iteration's counters, then per-job done counts)
+    // and an event per slot.
+    struct LoopPoll {
+        void* host = nullptr;
+        size_t slot_bytes = 0;
+        hipEvent_t ready[2] = {nullptr, nullptr};
+        ~LoopPoll() {
+            if (host) (void)hipHostFree(host);
+            for (hipEvent_t e : ready)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } poll;
 };

@@ -157,8 +157,9 @@ struct StageTimer {
 
 // One regenerating stream of `samples` samples per pixel.  Iteration i runs
 // regenerate(i) -> closest(i) -> shade(i) -> shadow(i); queue lengths live in
-// device counters, and the host checks for completion once every
-// kIterChunk iterations.
+// device counters.  The host enqueues chunks of kIterChunk iterations and
+// checks for completion one chunk behind the GPU, so the GPU never waits on
+// the host; a chunk enqueued after the loop drained finds empty queues.
 static constexpr uint32_t kIterChunk = 8;
 static constexpr uint32_t kCacheStreamMax = 8;
 
@@ -251,16 +252,45 @@ static void init_cache_overlap(rm_renderer* r) {
     for (hipEvent_t& e : o.done) RM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
-// Records the completion time of every job whose paths have all ended.
-static void batch_poll(BatchRun* br, hipStream_t st) {
+// Records the completion time of every job whose paths have all ended,
+// from per-job done counts already copied to the host.
+static void batch_poll_counts(BatchRun* br, const uint32_t* done) {
     const uint32_t n = br->view.num_jobs;
-    std::vector<uint32_t> done(n);
-    RM_HIP(hipMemcpyAsync(done.data(), br->view.done, n * 4, hipMemcpyDeviceToHost, st));
-    RM_HIP(hipStreamSynchronize(st));
     const double now =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
     for (uint32_t j = 0; j < n; j++)
         if ((*br->job_ms)[j] < 0.0 && done[j] >= (*br->job_paths)[j]) (*br->job_ms)[j] = now - br->t0_ms;
+}
+static void batch_poll(BatchRun* br, hipStream_t st) {
+    std::vector<uint32_t> done(br->view.num_jobs);
+    RM_HIP(hipMemcpyAsync(done.data(), br->view.done, done.size() * 4, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+    batch_poll_counts(br, done.data());
+}
+
+// The renderer's pinned poll slots, grown to `slot_bytes` each when needed.
+struct LoopPollRef {
+    char* host;
+    size_t slot_bytes;
+    hipEvent_t* ready;
+    IterCounters* probe(int slot) const { return reinterpret_cast<IterCounters*>(host + slot * slot_bytes); }
+    uint32_t* done(int slot) const {
+        return reinterpret_cast<uint32_t*>(host + slot * slot_bytes + sizeof(IterCounters));
+    }
+};
+static LoopPollRef loop_poll(rm_renderer* r, size_t slot_bytes) {
+    auto& p = r->poll;
+    if (!p.ready[0]) {
+        for (hipEvent_t& e : p.ready) RM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (p.slot_bytes < slot_bytes) {
+        if (p.host) RM_HIP(hipHostFree(p.host));
+        p.host = nullptr;
+        p.slot_bytes = 0;
+        RM_HIP(hipHostMalloc(&p.host, 2 * slot_bytes, hipHostMallocDefault));
+        p.slot_bytes = slot_bytes;
+    }
+    return LoopPollRef{static_cast<char*>(p.host), p.slot_bytes, p.ready};
 }
 
 static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* stats) {
@@ -272,8 +302,12 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     Counters* cnt = r->counters_buf.as<Counters>();
     float* image = r->image_buf.as<float>();
     const int cus = dev->props.multiProcessorCount;
-    const int tblocks = cus * dev->trace_blocks_per_cu;
-    const int ablocks = cus * dev->any_blocks_per_cu;
+    // the persistent grids: at most what fits (the spill rows are sized for it)
+    const uint32_t want = r->u.trace_blocks_per_cu;
+    const int tbpc = want ? std::min((int)want, dev->trace_blocks_per_cu) : dev->trace_blocks_per_cu;
+    const int abpc = want ? std::min((int)want, dev->any_blocks_per_cu) : dev->any_blocks_per_cu;
+    const int tblocks = cus * tbpc;
+    const int ablocks = cus * abpc;
     BatchRun* br = r->batch;
     hipFunction_t shade_fn = br ? s->jit.shade_batch : (r->cache_mode ? s->jit_cache.shade_cache : s->jit.shade);
     RM_REQUIRE(shade_fn != nullptr, "material module lacks the batch shading kernel");
@@ -319,6 +353,8 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     ca.spill = s->d_spill.as<uint2>();
     ca.counters = cnt;
     ca.camera = s->view.camera;
+    ca.tlas = s->two_level ? s->d_tlas.as<TlasInstance>() : nullptr;
+    ca.root2l = s->root2l;
     if (br) ca.batch = br->view;
 
     const bool sorting = r->u.material_sort != 0;
@@ -331,6 +367,7 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         so.keys = r->sort_buf.as<uint32_t>();
         so.order = so.keys + r->capacity;
         so.counters = cnt;
+        so.key_shift = std::min(r->u.sort_key_shift, 15u);
     }
     const int sort_blocks = cus * 4;
 
@@ -356,6 +393,8 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
     AnyArgs aa{};
     aa.nodes = ca.nodes;
     aa.tris = ca.tris;
+    aa.tlas = ca.tlas;
+    aa.root2l = ca.root2l;
     aa.shadows = r->shadows;
     aa.image = image;
     aa.spill = ca.spill;
@@ -456,9 +495,14 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         }
     } exec_guard{chunk_exec};
 
+    // pinned poll slots: the counters of the iteration after a chunk, then
+    // the per-job done counts of a batch
+    const uint32_t njobs = br ? br->view.num_jobs : 0u;
+    const size_t slot_bytes = (sizeof(IterCounters) + (size_t)njobs * 4 + 255) & ~(size_t)255;
+    LoopPollRef lp = loop_poll(r, slot_bytes);
     uint32_t iter = 0;
-    std::vector<IterCounters> host_it;
-    for (;;) {
+    uint32_t chunk_end[2] = {0, 0};
+    auto enqueue_chunk = [&](int slot) {
         RM_REQUIRE(iter + kIterChunk + 1 < (uint32_t)kMaxIters, "render stream exceeded the iteration limit");
         if (chunk_exec) {
             RM_HIP(hipGraphLaunch(chunk_exec, st));
@@ -466,12 +510,27 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
         } else {
             for (uint32_t c = 0; c < kIterChunk; c++, iter++) enqueue(iter);
         }
-        IterCounters probe{};
-        RM_HIP(hipMemcpyAsync(&probe, &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));
-        RM_HIP(hipStreamSynchronize(st));
-        if (br) batch_poll(br, st);
+        RM_HIP(hipMemcpyAsync(lp.probe(slot), &cnt->it[iter], sizeof(IterCounters), hipMemcpyDeviceToHost, st));
+        if (br) RM_HIP(hipMemcpyAsync(lp.done(slot), br->view.done, (size_t)njobs * 4, hipMemcpyDeviceToHost, st));
+        RM_HIP(hipEventRecord(lp.ready[slot], st));
+        chunk_end[slot] = iter;
+    };
+    std::vector<IterCounters> host_it;
+    int slot = 0;
+    enqueue_chunk(0);
+    for (;;) {
+        enqueue_chunk(slot ^ 1);  // runs while the host reads the previous chunk's counters
+        RM_HIP(hipEventSynchronize(lp.ready[slot]));
+        if (br) batch_poll_counts(br, lp.done(slot));
+        const IterCounters& probe = *lp.probe(slot);
         if (probe.generated >= total && probe.appended == 0) break;
+        slot ^= 1;
     }
+    // the chunk after the last one held no work: the stream's statistics and
+    // stage timers end with the chunk that drained it
+    iter = chunk_end[slot];
+    for (auto* v : {&tm.closest, &tm.sort, &tm.shade, &tm.any})
+        if (v->size() > iter) v->resize(iter);
     uint32_t cache_ctr[2] = {0, 0};  // lookups, training vertices
     if (cached) {
         RM_HIP(hipMemcpyAsync(cache_ctr, &cnt->cache_queries, 8, hipMemcpyDeviceToHost, st));

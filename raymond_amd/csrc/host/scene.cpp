@@ -4,11 +4,14 @@
 // Restates the device-facing parts of host/scene/ShapeBuilder.swift:251-340,
 // EntityBuilder.swift:79-173, LightBuilder.swift:83-281 and
 // host/lights/distribution.m:7-39.
+#include <tuple>
+#include <map>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 
+#include "../device/bvh2l.h"
 #include "../kernels/misc_api.h"
 #include "../kernels/trace_api.h"
 #include "../device/shade_kernels_abi.h"
@@ -273,10 +276,163 @@ void scene_build_accel(rm_scene* s, int threads) {
     s->d_spill.alloc(s->spill_threads * (size_t)(kMaxTraceStack - kLdsStack) * sizeof(uint2));
     RM_HIP(hipStreamSynchronize(st));
     s->accel_built = true;
+    s->two_level = false;
+    s->root2l = 0;
+    s->d_tlas.release();
     s->stats.triangles = tris.size();
+    s->stats.stored_triangles = tris.size();
+    s->stats.two_level = 0;
     s->stats.bvh_nodes = b.nodes.size();
     s->stats.bvh_depth = (uint32_t)b.depth;
     s->stats.bvh_build_seconds = b.build_seconds;
+}
+
+// Two-level form: a BVH8 per shape (distinct vertex and face range) over its
+// object-space triangles, then one over the instances' world boxes with one
+// instance per leaf, whose leaf records hold the world-to-object rows.  The
+// world box of an instance is its shape's object box mapped in double and
+// widened by 1e-5 of its largest coordinate, so that it holds every hit the
+// object-space tests can report for a ray through it.
+TwoLevelBuild build_two_level(const std::vector<float>& vertices, const std::vector<uint32_t>& indices,
+                              const std::vector<rm_instance_desc>& insts, int threads) {
+    const auto t0 = std::chrono::steady_clock::now();
+    TwoLevelBuild out;
+    std::string err;
+    const size_t ni = insts.size();
+    struct Shape {
+        uint32_t vo, fo, fc, root;
+        double lo[3], hi[3];
+    };
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t> shape_id;
+    std::vector<Shape> shapes;
+    std::vector<uint32_t> inst_shape(ni);
+    for (size_t k = 0; k < ni; k++) {
+        const rm_instance_desc& id = insts[k];
+        const auto key = std::make_tuple(id.vertex_offset, id.face_offset, id.face_count);
+        auto it = shape_id.find(key);
+        if (it == shape_id.end()) {
+            it = shape_id.emplace(key, (uint32_t)shapes.size()).first;
+            shapes.push_back(Shape{id.vertex_offset, id.face_offset, id.face_count, 0, {0, 0, 0}, {0, 0, 0}});
+        }
+        inst_shape[k] = it->second;
+    }
+    int depth = 0;
+    for (Shape& sh : shapes) {
+        std::vector<BuildTri> bt(sh.fc);
+        for (int a = 0; a < 3; a++) {
+            sh.lo[a] = INFINITY;
+            sh.hi[a] = -INFINITY;
+        }
+        for (uint32_t f = 0; f < sh.fc; f++) {
+            const uint32_t* ix = &indices[3 * ((size_t)sh.fo + f)];
+            vec3* v[3] = {&bt[f].v0, &bt[f].v1, &bt[f].v2};
+            for (int j = 0; j < 3; j++) {
+                const float* p = &vertices[3 * ((size_t)sh.vo + ix[j])];
+                *v[j] = vec3(p[0], p[1], p[2]);
+                for (int a = 0; a < 3; a++) {
+                    sh.lo[a] = std::min(sh.lo[a], (double)p[a]);
+                    sh.hi[a] = std::max(sh.hi[a], (double)p[a]);
+                }
+            }
+            bt[f].inst = 0;
+            bt[f].prim = f;
+        }
+        Bvh8Build b = build_bvh8(bt, threads, &err, 3);
+        RM_REQUIRE(err.empty(), "BVH build failed: " + err);
+        const uint32_t nbase = (uint32_t)out.nodes.size(), tbase = (uint32_t)out.tris.size();
+        for (Bvh8Node& n : b.nodes) {
+            n.child_base += nbase;
+            n.tri_base += tbase;
+        }
+        sh.root = nbase;
+        out.nodes.insert(out.nodes.end(), b.nodes.begin(), b.nodes.end());
+        out.tris.insert(out.tris.end(), b.tris.begin(), b.tris.end());
+        depth = std::max(depth, b.depth);
+        out.stored += sh.fc;
+    }
+    // top level: each instance with geometry as the box (lo, hi, hi)
+    std::vector<BuildTri> boxes;
+    std::vector<mat4> xf(ni);
+    for (size_t k = 0; k < ni; k++) {
+        const Shape& sh = shapes[inst_shape[k]];
+        out.total += sh.fc;
+        xf[k] = mat4_from(insts[k].transform);
+        if (sh.fc == 0) continue;
+        const mat4& M = xf[k];
+        const double m[3][4] = {{M.c0.x, M.c1.x, M.c2.x, M.c3.x}, {M.c0.y, M.c1.y, M.c2.y, M.c3.y},
+                                {M.c0.z, M.c1.z, M.c2.z, M.c3.z}};
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; c++) {
+            const double p[3] = {(c & 1) ? sh.hi[0] : sh.lo[0], (c & 2) ? sh.hi[1] : sh.lo[1],
+                                 (c & 4) ? sh.hi[2] : sh.lo[2]};
+            for (int a = 0; a < 3; a++) {
+                const double w = m[a][0] * p[0] + m[a][1] * p[1] + m[a][2] * p[2] + m[a][3];
+                lo[a] = std::min(lo[a], w);
+                hi[a] = std::max(hi[a], w);
+            }
+        }
+        double mag = 0.0;
+        for (int a = 0; a < 3; a++) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+        const double pad = 1e-5 * mag + 1e-30;
+        BuildTri t;
+        float flo[3], fhi[3];
+        for (int a = 0; a < 3; a++) {
+            flo[a] = (float)(lo[a] - pad);
+            fhi[a] = (float)(hi[a] + pad);
+            if ((double)flo[a] > lo[a] - pad) flo[a] = std::nextafter(flo[a], -INFINITY);
+            if ((double)fhi[a] < hi[a] + pad) fhi[a] = std::nextafter(fhi[a], INFINITY);
+        }
+        t.v0 = vec3(flo[0], flo[1], flo[2]);
+        t.v1 = vec3(fhi[0], fhi[1], fhi[2]);
+        t.v2 = t.v1;
+        t.inst = (uint32_t)k;
+        t.prim = 0;
+        boxes.push_back(t);
+    }
+    Bvh8Build top = build_bvh8(boxes, threads, &err, 1);
+    RM_REQUIRE(err.empty(), "top-level BVH build failed: " + err);
+    // one stack entry per level of both trees and the instance's marker
+    RM_REQUIRE(depth + top.depth + 2 < kMaxTraceStack, "two-level tree too deep for the traversal stack");
+    const uint32_t nbase = (uint32_t)out.nodes.size();
+    for (Bvh8Node& n : top.nodes) n.child_base += nbase;
+    out.nodes.insert(out.nodes.end(), top.nodes.begin(), top.nodes.end());
+    out.recs.resize(top.tris.size());
+    for (size_t i = 0; i < top.tris.size(); i++) {
+        const uint32_t k = top.tris[i].inst;
+        TlasInstance& r = out.recs[i];
+        std::memset(&r, 0, sizeof(r));
+        RM_REQUIRE(tlas_inverse(xf[k], r.m), "instance " + std::to_string(k) + " has a singular transform");
+        r.blas_root = shapes[inst_shape[k]].root;
+        r.inst = k;
+    }
+    // valid pointers for scenes without geometry
+    if (out.recs.empty()) out.recs.resize(1);
+    if (out.tris.empty()) out.tris.resize(1);
+    out.root = nbase;
+    out.depth = depth + top.depth;
+    out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return out;
+}
+
+void scene_build_accel_two_level(rm_scene* s, int threads) {
+    const TwoLevelBuild b = build_two_level(s->vertices, s->indices, s->inst_desc, threads);
+    hipStream_t st = s->dev->stream;
+    s->d_nodes.upload(b.nodes.data(), b.nodes.size() * sizeof(Bvh8Node), st);
+    s->d_tris.upload(b.tris.data(), b.tris.size() * sizeof(BvhTri), st);
+    s->d_tlas.upload(b.recs.data(), b.recs.size() * sizeof(TlasInstance), st);
+    const int bpc = std::max(s->dev->trace_blocks_per_cu, s->dev->any_blocks_per_cu);
+    s->spill_threads = (size_t)s->dev->props.multiProcessorCount * (size_t)bpc * kTraceBlock;
+    s->d_spill.alloc(s->spill_threads * (size_t)(kMaxTraceStack - kLdsStack) * sizeof(uint2));
+    RM_HIP(hipStreamSynchronize(st));
+    s->accel_built = true;
+    s->two_level = true;
+    s->root2l = b.root;
+    s->stats.triangles = b.total;
+    s->stats.stored_triangles = b.stored;
+    s->stats.two_level = 1;
+    s->stats.bvh_nodes = b.nodes.size();
+    s->stats.bvh_depth = (uint32_t)b.depth;
+    s->stats.bvh_build_seconds = b.seconds;
 }
 
 // Compiles (or takes from the code-object cache) and loads one module.
@@ -384,6 +540,8 @@ void trace_closest_raw(rm_scene* s, const float* rays, uint32_t n, float* t, uin
     ClosestArgs a{};
     a.nodes = s->d_nodes.as<Bvh8Node>();
     a.tris = s->d_tris.as<BvhTri>();
+    a.tlas = s->two_level ? s->d_tlas.as<TlasInstance>() : nullptr;
+    a.root2l = s->root2l;
     a.count = n;
     a.fetch = &cnt.as<Counters>()->it[0].fetch_closest;
     a.paths.ox = f[0]; a.paths.oy = f[1]; a.paths.oz = f[2];
@@ -427,6 +585,8 @@ void trace_any_raw(rm_scene* s, const float* rays, uint32_t n, uint32_t* occlude
     AnyArgs a{};
     a.nodes = s->d_nodes.as<Bvh8Node>();
     a.tris = s->d_tris.as<BvhTri>();
+    a.tlas = s->two_level ? s->d_tlas.as<TlasInstance>() : nullptr;
+    a.root2l = s->root2l;
     a.count = n;
     a.fetch = &cnt.as<Counters>()->it[0].fetch_any;
     a.shadows.ox = f[0]; a.shadows.oy = f[1]; a.shadows.oz = f[2];

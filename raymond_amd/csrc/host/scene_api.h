@@ -3,6 +3,7 @@
 #include <string>
 #include <vector>
 
+#include "../device/bvh2l.h"
 #include "bvh_build.h"
 #include "core.h"
 
@@ -15,6 +16,21 @@ std::vector<BuildTri> world_triangles(const rm_scene* s);
 void scene_create(rm_device* dev, const rm_scene_desc* d, rm_scene* s);
 void scene_set_camera(rm_scene* s, const rm_camera_desc* c);
 void scene_build_accel(rm_scene* s, int threads);
+void scene_build_accel_two_level(rm_scene* s, int threads);
+
+// The two-level form's arrays (device/bvh2l.h): the shapes' trees, then the
+// top-level tree at `root`; triangles of the shapes; instance records.
+struct TwoLevelBuild {
+    std::vector<Bvh8Node> nodes;
+    std::vector<BvhTri> tris;
+    std::vector<TlasInstance> recs;
+    uint32_t root = 0;
+    int depth = 0;
+    uint64_t stored = 0, total = 0;  // triangles stored once per shape, and over all instances
+    double seconds = 0;
+};
+TwoLevelBuild build_two_level(const std::vector<float>& vertices, const std::vector<uint32_t>& indices,
+                              const std::vector<rm_instance_desc>& insts, int threads);
 void scene_compile_materials(rm_scene* s, const char* src, size_t len, const char* options, std::string& log);
 void scene_build_envmap(rm_scene* s);
 // Loads the scene's material library with the radiance-cache shading kernel.

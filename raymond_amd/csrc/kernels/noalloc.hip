@@ -78,6 +78,8 @@ void trace_closest_noalloc(rm_scene* sc, const float* rays, uint32_t n, float* t
     float* cols = reinterpret_cast<float*>(s + kNoAllocHeadWords);
     ClosestArgs a{};
     a.nodes = sc->d_nodes.as<Bvh8Node>();
+    a.tlas = sc->two_level ? sc->d_tlas.as<TlasInstance>() : nullptr;
+    a.root2l = sc->root2l;
     a.tris = sc->d_tris.as<BvhTri>();
     a.count = n;
     a.fetch = s;

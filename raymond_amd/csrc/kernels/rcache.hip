@@ -22,7 +22,14 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float acc16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kInferWaves = 4;
-constexpr int kTrainWaves = 8;
+// Waves per training workgroup.  A step of 8192 samples is 256 waves; the
+// kernel waits on its table-gradient atomics, so more, smaller workgroups
+// keep more of them in flight: per step 119.8 us at 8 waves, 95.7 at 4,
+// 84.4 at 2, 84.0 at 1 (2 has the lower LDS cost per wave of weights).
+#ifndef RM_TRAIN_WAVES
+#define RM_TRAIN_WAVES 2
+#endif
+constexpr int kTrainWaves = RM_TRAIN_WAVES;
 
 // LDS row strides (halves), padded against bank conflicts; multiples of 4 so
 // every 4-half operand read is one aligned 8-byte load.
@@ -84,26 +91,33 @@ __device__ inline float half_hi(uint32_t w) {
     return (float)h;
 }
 
-__device__ void load_fwd_weights(RcLdsFwd& W, const uint16_t* bits) {
-    const _Float16* mlp = reinterpret_cast<const _Float16*>(bits);
-    for (int i = threadIdx.x; i < kRcW1; i += (int)blockDim.x) {
-        const int in = i / kRcHidden, out = i % kRcHidden;
-        W.w1t[out * kS1T + in] = mlp[i];
-    }
-    for (int i = threadIdx.x; i < kRcW2; i += (int)blockDim.x) {
-        const int in = i / kRcHidden, out = i % kRcHidden;
-        W.w2t[out * kS2 + in] = mlp[kOffW2 + i];
-    }
-    for (int i = threadIdx.x; i < kRcW3; i += (int)blockDim.x) {
-        const int in = i / kRcW3Cols, out = i % kRcW3Cols;
-        W.w3t[out * kS3T + in] = mlp[kOffW3 + i];
+// Calls f(i, w[i]) for the `count` FP16 values from w + offset, read 8 at a
+// time (one 16-byte load per lane; offsets and counts are multiples of 8).
+template <class F>
+__device__ inline void for_each_half8(const uint16_t* bits, int offset, int count, F f) {
+    const uint4* src = reinterpret_cast<const uint4*>(bits + offset);
+    for (int j = threadIdx.x; j < count / 8; j += (int)blockDim.x) {
+        const uint4 v = src[j];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint16_t b = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+            _Float16 h;
+            __builtin_memcpy(&h, &b, 2);
+            f(8 * j + e, h);
+        }
     }
 }
+
+__device__ void load_fwd_weights(RcLdsFwd& W, const uint16_t* bits) {
+    for_each_half8(bits, 0, kRcW1, [&](int i, _Float16 h) { W.w1t[(i % kRcHidden) * kS1T + i / kRcHidden] = h; });
+    for_each_half8(bits, kOffW2, kRcW2, [&](int i, _Float16 h) { W.w2t[(i % kRcHidden) * kS2 + i / kRcHidden] = h; });
+    for_each_half8(bits, kOffW3, kRcW3, [&](int i, _Float16 h) { W.w3t[(i % kRcW3Cols) * kS3T + i / kRcW3Cols] = h; });
+}
 __device__ void load_bwd_weights(RcLdsBwd& W, const uint16_t* bits) {
-    const _Float16* mlp = reinterpret_cast<const _Float16*>(bits);
-    for (int i = threadIdx.x; i < kRcW1; i += (int)blockDim.x) W.w1[(i / kRcHidden) * kS1 + i % kRcHidden] = mlp[i];
-    for (int i = threadIdx.x; i < kRcW2; i += (int)blockDim.x) W.w2[(i / kRcHidden) * kS2 + i % kRcHidden] = mlp[kOffW2 + i];
-    for (int i = threadIdx.x; i < kRcW3; i += (int)blockDim.x) W.w3[(i / kRcW3Cols) * kS3 + i % kRcW3Cols] = mlp[kOffW3 + i];
+    for_each_half8(bits, 0, kRcW1, [&](int i, _Float16 h) { W.w1[(i / kRcHidden) * kS1 + i % kRcHidden] = h; });
+    for_each_half8(bits, kOffW2, kRcW2, [&](int i, _Float16 h) { W.w2[(i / kRcHidden) * kS2 + i % kRcHidden] = h; });
+    for_each_half8(bits, kOffW3, kRcW3, [&](int i, _Float16 h) { W.w3[(i / kRcW3Cols) * kS3 + i % kRcW3Cols] = h; });
 }
 
 // Corner c (bits x, y, z) of the cell around p at `level`: table index and

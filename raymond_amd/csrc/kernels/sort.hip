@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kSortBlock) void sort_keys_kernel(SortArgs a) {
         uint32_t key = kSortBins - 1;  // miss: the world shader
         if (a.hits.t[i] > 0.0f) {
             const uint32_t face = a.instances[a.hits.inst[i]].face_offset + a.hits.prim[i];
-            const uint32_t m = a.materials[face];
+            const uint32_t m = (uint32_t)a.materials[face] >> a.key_shift;
             key = m < (uint32_t)(kSortBins - 2) ? m : (uint32_t)(kSortBins - 2);
         }
         a.keys[i] = key;

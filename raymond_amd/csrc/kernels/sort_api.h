@@ -18,6 +18,7 @@ struct SortArgs {
     Counters* counters;
     uint32_t iter;
     const uint32_t* iter_dev;   // captured bounce loops (iteration_of)
+    uint32_t key_shift;         // granularity: materials m >> key_shift share a bin
 };
 
 // keys(iter) then scatter(iter); both read the queue length from the

@@ -14,6 +14,7 @@
 //  * traversal stack: the bottom kLdsStack entries per lane in LDS (column
 //    per lane), deeper entries spill to a per-thread global array.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "../device/common.h"
 #include "../device/rng.h"
 #include "../device/isect.h"
@@ -75,15 +76,27 @@ struct RayState {
     float bu, bv, bdet;  // closest hit's scaled barycentrics and their sum
     bool found;
     uint2 cur;  // node group: child base, (remaining child hits << 24) | internal mask
-    uint2 tg;   // postponed leaf triangles: base, mask
+    uint2 tg;   // postponed leaf triangles: base, mask (top level of a two-level tree: instances)
     int sp;
     uint32_t n_nodes, n_tris;
+    // two-level trees: the world ray, the instance entered, in the top level
+    vec3 wo, wd;
+    uint32_t inst;
+    bool top;
 
-    __device__ void init(uint32_t root, vec3 org, vec3 d, float t0, float t1) {
+    __device__ void set_ray(vec3 org, vec3 d) {
         o = org;
         inv = safe_inv_dir(d);
         sh = make_shear(d);
         oct = dir_octant(d);
+    }
+
+    __device__ void init(uint32_t root, vec3 org, vec3 d, float t0, float t1) {
+        set_ray(org, d);
+        wo = org;
+        wd = d;
+        inst = 0;
+        top = true;
         // a one-child pseudo node whose only child is the root
         cur = make_uint2(root, (1u << (24 + oct)) | 1u);
         sp = 0;
@@ -105,8 +118,44 @@ struct RayState {
     // that no step is a bare pop), or the postponed triangles of the leaves
     // hit so far.  Returns true when the traversal has ended (ANY: also at
     // the first hit in (tmin, tmax)).
-    template <bool ANY, bool STATS>
-    __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris, const StackRef& stk) {
+    // Two-level trees: enters the next instance of the top-level leaf set in
+    // tg.  The top-level group (if it has hits left) and a marker entry with
+    // the set's other instances go on the stack; popping the marker returns
+    // to the world ray.  The ray is mapped affinely into the instance's
+    // object space without renormalising the direction, so t is the same
+    // parameter on both sides, and the shape's tree is visited from its root.
+    __device__ void enter(const TlasInstance* __restrict__ insts, const StackRef& stk) {
+        const int b = __builtin_ctz(tg.y);
+        tg.y &= tg.y - 1u;
+        const uint4* ip = reinterpret_cast<const uint4*>(insts + tg.x + (uint32_t)b);
+        const uint4 r0 = ip[0], r1 = ip[1], r2 = ip[2], r3 = ip[3];
+        if (cur.y & 0xff000000u) {
+            stk.store(sp, cur);
+            sp++;
+        }
+        stk.store(sp, make_uint2(tg.x, kMarker2l | tg.y));
+        sp++;
+        TlasInstance ti;
+        const uint4 rows[3] = {r0, r1, r2};
+        for (int j = 0; j < 3; j++) {
+            ti.m[4 * j + 0] = __uint_as_float(rows[j].x);
+            ti.m[4 * j + 1] = __uint_as_float(rows[j].y);
+            ti.m[4 * j + 2] = __uint_as_float(rows[j].z);
+            ti.m[4 * j + 3] = __uint_as_float(rows[j].w);
+        }
+        set_ray(tlas_point(ti, wo), tlas_vector(ti, wd));
+        inst = r3.y;
+        top = false;
+        cur = make_uint2(r3.x, (1u << (24 + oct)) | 1u);
+        tg = make_uint2(0u, 0u);
+    }
+
+    template <bool ANY, bool STATS, bool TWO = false>
+    __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris,
+                         const TlasInstance* __restrict__ insts, const StackRef& stk) {
+        // A top-level leaf set holds instances, entered right away (one per
+        // step), so that tg only ever holds triangles below.
+        if (TWO && top && tg.y) enter(insts, stk);
         // Leaf triangles wait until at least a quarter of the wave's active
         // lanes have some (or no lane can take a node), then those lanes test
         // a few each together instead of each node step running a loop most
@@ -117,13 +166,23 @@ struct RayState {
         const unsigned long long want = __ballot(has_tris);
         if (__popcll(want) * 4 >= __popcll(__ballot(true)) || __ballot(node_work && !has_tris) == 0ull) {
             if (!has_tris) return !node_work;
-            return test_tris<ANY, STATS>(tris) || (!node_work && tg.y == 0u);
+            return test_tris<ANY, STATS, TWO>(tris) || (!node_work && tg.y == 0u);
         }
         if (has_tris) return false;
         if (!(cur.y & 0xff000000u)) {
             if (sp == 0) return true;
             sp--;
-            cur = stk.load(sp);  // pushed groups always hold a hit child
+            const uint2 e = stk.load(sp);
+            if (TWO && (e.y & kMarker2l)) {
+                // back in the world: the set's other instances, then the
+                // top-level groups below the marker
+                set_ray(wo, wd);
+                top = true;
+                tg = make_uint2(e.x, e.y & ~kMarker2l);
+                cur = make_uint2(0u, 0u);
+                return false;
+            }
+            cur = e;  // pushed groups always hold a hit child
         }
         const uint32_t hits = cur.y >> 24;
         const int k = __builtin_ctz(hits);
@@ -217,7 +276,7 @@ struct RayState {
     // sphere field (Mrays/s): 1 per phase 2841, 2 per phase 2909, 3 2891,
     // 4 2870; emptying every lane's list per phase was slower than 1.
     static constexpr int kTrisPerPhase = 2;
-    template <bool ANY, bool STATS>
+    template <bool ANY, bool STATS, bool TWO>
     __device__ bool test_tris(const BvhTri* __restrict__ tris) {
         for (int k = 0; k < kTrisPerPhase && tg.y; k++) {
             const int b = __builtin_ctz(tg.y);
@@ -229,10 +288,12 @@ struct RayState {
             const vec3 v1(__uint_as_float(bq.x), __uint_as_float(bq.y), __uint_as_float(bq.z));
             const vec3 v2(__uint_as_float(cq.x), __uint_as_float(cq.y), __uint_as_float(cq.z));
             float t, u, v, det;
+            // a two-level tree's triangles are the shape's: the instance is the one entered
+            const uint32_t ti = TWO ? inst : a.w;
             if (intersect_triangle_uvd(o, sh, v0, v1, v2, tmin, tmax, t, u, v, det) &&
-                hit_before(t, a.w, bq.w, best_t, best_inst, best_prim)) {
+                hit_before(t, ti, bq.w, best_t, best_inst, best_prim)) {
                 best_t = t;
-                best_inst = a.w;
+                best_inst = ti;
                 best_prim = bq.w;
                 bu = u;
                 bv = v;
@@ -334,7 +395,7 @@ struct RayPool {
 // Closest hit over a path queue.  Camera paths (depth 0) use the camera's
 // clip range, continuation rays (eps, inf) (intersection.metal:201-202);
 // raw batches bring their own ranges.
-template <bool STATS>
+template <bool STATS, bool TWO>
 __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
@@ -362,9 +423,9 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
             i = pool.ray;
             const vec3 o(q.ox[i], q.oy[i], q.oz[i]);
             const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
-            uint32_t root = 0;
+            uint32_t root = TWO ? a.root2l : 0u;
             float near_clip = a.camera.near_clip, far_clip = a.camera.far_clip;
-            if (a.batch.jobs) {
+            if (!TWO && a.batch.jobs) {
                 const JobInfo& job = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])];
                 root = job.root;
                 near_clip = job.camera.near_clip;
@@ -384,7 +445,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs 
             active = true;
         }
         if (!__any(active)) break;
-        if (active && s.step<false, STATS>(a.nodes, a.tris, stk)) {
+        if (active && s.step<false, STATS, TWO>(a.nodes, a.tris, a.tlas, stk)) {
             a.hits.t[i] = s.found ? s.best_t : -1.0f;
             a.hits.prim[i] = s.best_prim;
             a.hits.inst[i] = s.best_inst;
@@ -483,7 +544,7 @@ __global__ void regenerate_kernel(RegenArgs a) {
 // Any-hit over the shadow queue.  With accumulate, an unoccluded ray adds
 // (weight, 1) to its pixel (shadow.metal:6-29); otherwise the occlusion flag
 // is written to `occluded`.
-template <bool STATS>
+template <bool STATS, bool TWO>
 __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
@@ -512,13 +573,13 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
             const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
             const float tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
             const float tmax = q.tmax[i];
-            uint32_t root = 0;
-            if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
+            uint32_t root = TWO ? a.root2l : 0u;
+            if (!TWO && a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
             s.init(root, o, d, tmin, tmax);
             active = true;
         }
         if (!__any(active)) break;
-        if (active && s.step<true, STATS>(a.nodes, a.tris, stk)) {
+        if (active && s.step<true, STATS, TWO>(a.nodes, a.tris, a.tlas, stk)) {
             const bool hit = s.found;
             if (a.image) {
                 if (!hit) {
@@ -559,15 +620,50 @@ __global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
 
 namespace rm {
 
+namespace {
+// The two-level kernels hold a few more registers than the flat ones; their
+// persistent grid shrinks with their occupancy, so that every wave of it is
+// resident at once.  Callers size grids (and the spill rows) for the flat
+// kernels.
+int two_level_blocks(int blocks, bool any) {
+    static const int ratio[2][2] = {
+        {trace_blocks_per_cu(false), [] {
+             int n = 0;
+             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_closest_kernel<false, true>, kTraceBlock, 0);
+             return n;
+         }()},
+        {trace_blocks_per_cu(true), [] {
+             int n = 0;
+             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_any_kernel<false, true>, kTraceBlock, 0);
+             return n;
+         }()}};
+    const int flat = ratio[any][0], two = ratio[any][1];
+    if (flat <= 0 || two <= 0 || two >= flat) return blocks;
+    return std::max(1, (int)((long long)blocks * two / flat));
+}
+}  // namespace
+
 hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream) {
-    if (stats) hipLaunchKernelGGL(trace_closest_kernel<true>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
-    else hipLaunchKernelGGL(trace_closest_kernel<false>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+    if (a.tlas) {
+        blocks = two_level_blocks(blocks, false);
+        if (stats) hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_closest_kernel<true, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_closest_kernel<false, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+        return hipGetLastError();
+    }
+    if (stats) hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_closest_kernel<true, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_closest_kernel<false, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_any(const AnyArgs& a, int blocks, bool stats, hipStream_t stream) {
-    if (stats) hipLaunchKernelGGL(trace_any_kernel<true>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
-    else hipLaunchKernelGGL(trace_any_kernel<false>, dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+    if (a.tlas) {
+        blocks = two_level_blocks(blocks, true);
+        if (stats) hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_any_kernel<true, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_any_kernel<false, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+        return hipGetLastError();
+    }
+    if (stats) hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_any_kernel<true, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(trace_any_kernel<false, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -578,8 +674,8 @@ hipError_t launch_regenerate(const RegenArgs& a, int blocks, hipStream_t stream)
 
 int trace_blocks_per_cu(bool any) {
     int n = 0;
-    if (any) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_any_kernel<false>, kTraceBlock, 0);
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_closest_kernel<false>, kTraceBlock, 0);
+    if (any) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_any_kernel<false, false>, kTraceBlock, 0);
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_closest_kernel<false, false>, kTraceBlock, 0);
     return n;
 }
 

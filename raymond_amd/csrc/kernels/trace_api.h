@@ -5,6 +5,7 @@
 #include "../device/queues.h"
 #include "../device/scene.h"
 #include "../device/batch.h"
+#include "../device/bvh2l.h"
 #include "../device/camera.h"
 
 namespace rm {
@@ -38,6 +39,9 @@ struct ClosestArgs {
     // captured bounce loops: count and fetch come from counters->it[iteration_of(iter_dev, iter)]
     const uint32_t* iter_dev;
     uint32_t iter;
+    // two-level scenes: instance records of the top-level tree (null: flat)
+    const TlasInstance* tlas;
+    uint32_t root2l;
 };
 
 struct RegenArgs {
@@ -76,6 +80,8 @@ struct AnyArgs {
     BatchView batch;
     const uint32_t* iter_dev;   // captured bounce loops, as in ClosestArgs
     uint32_t iter;
+    const TlasInstance* tlas;   // two-level scenes, as in ClosestArgs
+    uint32_t root2l;
 };
 
 hipError_t launch_trace_closest(const ClosestArgs& a, int blocks, bool stats, hipStream_t stream);

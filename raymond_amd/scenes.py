@@ -631,6 +631,46 @@ def sphere_field(width=1920, height=1080, grid=8, level=6, radius=0.4, amplitude
     return d
 
 
+def instanced_field(width=1920, height=1080, grid=16, shapes=4, level=5, radius=0.4, amplitude=0.05):
+    """grid x grid instances of `shapes` displaced icospheres (seeds 1000+i),
+    each instance a turn about z (seed 9), a scale of 0.6-1.4 and a jittered
+    grid position (seed 7); Principled per shape (seed 42), ground plane,
+    analytic sky.  Built two-level, each shape's triangles are stored once."""
+    d = SceneDescription("instances")
+    rng_mat = np.random.default_rng(42)
+    mats = []
+    for i in range(shapes):
+        base = rng_mat.uniform(0, 1, 3)
+        mats.append(principled_material("shape %02d" % i, Base_Color=[float(x) for x in base] + [1.0],
+                                        Metallic=1.0 if i % 3 == 1 else 0.0,
+                                        Roughness=float(rng_mat.uniform(0.05, 1.0)), Specular=0.5,
+                                        Clearcoat=float(rng_mat.uniform(0, 1))))
+    mats.append(diffuse_material("ground", (0.5, 0.5, 0.5)))
+    d.surface_materials = mats
+    d.light_materials = [sky_world()]
+    d.world_shader = 0
+    v0, f0 = icosphere(level)
+    ids = []
+    for i in range(shapes):
+        v = v0 * (radius * (1.0 + amplitude * value_noise(v0, 1000 + i) / radius))[:, None]
+        ids.append(d.add_shape(Shape(v, vertex_normals(v, f0), f0, i)))
+    rng = np.random.default_rng(9)
+    jitter = np.random.default_rng(7).uniform(-0.25, 0.25, (grid * grid, 2))
+    spacing = 8.0 / grid
+    for i in range(grid * grid):
+        gx, gy = i % grid, i // grid
+        c = np.array([(gx - (grid - 1) / 2.0) * spacing + jitter[i, 0] * spacing,
+                      (gy - (grid - 1) / 2.0) * spacing + jitter[i, 1] * spacing, 0.0])
+        sc = rng.uniform(0.6, 1.4) * spacing
+        c[2] = radius * sc
+        d.add_entity(ids[i % shapes], translate(c) @ rotate_z(rng.uniform(0, 360)) @ scale((sc, sc, sc)))
+    g = 50.0
+    d.add_entity(d.add_shape(merge([quad((-g, -g, 0), (g, -g, 0), (g, g, 0), (-g, g, 0), shapes)])))
+    d.camera = dict(transform=look_at((0.0, -8.5, 5.5), (0.0, 0.3, 0.0)), near=1e-3, far=1e4, focal=2.0)
+    d.width, d.height = width, height
+    return d
+
+
 def singlet_lens(focal_mm=50.0, f_number=4.0):
     """A biconvex N-BK7 singlet (n_d 1.5168, V_d 64.17) with a stop behind it,
     as lens surfaces for api.lens_desc: object medium, front and back faces,
@@ -646,11 +686,12 @@ def singlet_lens(focal_mm=50.0, f_number=4.0):
                      [0.0, 0.0, 100.0, 1.0, 0.0]], np.float32)
 
 
-def build_device_scene(device, desc, threads=0, envmap=True, noinline=None):
-    """SceneLoader.loadScene: materials (JIT), shapes/entities (upload + BVH), lights (envmap)."""
+def build_device_scene(device, desc, threads=0, envmap=True, noinline=None, two_level=False):
+    """SceneLoader.loadScene: materials (JIT), shapes/entities (upload + BVH,
+    flat or two-level), lights (envmap)."""
     scene = api.Scene(device, desc)
     scene.compile_materials(desc.material_source(noinline))
-    scene.build_accel(threads)
+    scene.build_accel(threads, two_level=two_level)
     if envmap:
         scene.build_envmap()
     return scene

@@ -418,3 +418,20 @@ def xyz_to_rgb(xyz, L=None):
     out = np.zeros_like(xyz)
     L.shim_xyz_to_rgb(_p(xyz), _p(out), xyz.shape[0])
     return out
+
+
+def two_level_brute(vertices, faces, ranges, transforms, rays, any_hit=False, L=None):
+    """Closest (or any) hits by exhaustion over instances (vertex offset,
+    face offset, face count) with column-major object-to-world transforms."""
+    L = L or lib()
+    v, r, xf = f32(vertices).reshape(-1, 3), f32(rays).reshape(-1, 8), f32(transforms).reshape(-1, 16)
+    fa = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+    rg = np.ascontiguousarray(ranges, np.uint32).reshape(-1, 3)
+    n = r.shape[0]
+    t = np.zeros(n, np.float32)
+    inst = np.zeros(n, np.uint32)
+    prim = np.zeros(n, np.uint32)
+    bary = np.zeros((n, 2), np.float32)
+    L.shim_two_level_brute(_p(v), _p(fa), _p(rg), _p(xf), rg.shape[0], _p(r), n, int(bool(any_hit)), _p(t), _p(inst),
+                           _p(prim), _p(bary))
+    return t, inst, prim, bary

@@ -1,6 +1,7 @@
 // Host build of the shared device library for CPU property tests: exposes
 // RNG, elementary functions, warps, BSDF lobes, lights, the triangle test and
 // the lens camera through plain C entry points.  Test infrastructure only.
+#include "bvh2l.h"
 #include "camera.h"
 #include "isect.h"
 #include "material_prelude.h"
@@ -581,6 +582,59 @@ void shim_xyz_to_rgb(const float* xyz, float* rgb, int n) {
     for (int i = 0; i < n; i++) {
         const vec3 c = xyz_to_rgb(vec3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
         rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
+    }
+}
+
+// Two-level hits by exhaustion: every instance (vertex offset, face offset,
+// face count, 16 floats of column-major object-to-world transform), the ray
+// mapped by its world-to-object rows, every triangle of its shape.
+void shim_two_level_brute(const float* vertices, const uint32_t* faces, const uint32_t* inst_ranges,
+                          const float* transforms, int ni, const float* rays, int n, int any, float* t,
+                          uint32_t* inst, uint32_t* prim, float* bary) {
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        const vec3 wo(r[0], r[1], r[2]), wd(r[4], r[5], r[6]);
+        float bt = r[7];
+        uint32_t bi = 0xffffffffu, bp = 0xffffffffu;
+        float bu = 0.0f, bv = 0.0f, bd = 1.0f;
+        bool found = false;
+        for (int k = 0; k < ni && !(any && found); k++) {
+            const float* m = transforms + 16 * k;
+            mat4 M;
+            M.c0 = vec4(m[0], m[1], m[2], m[3]);
+            M.c1 = vec4(m[4], m[5], m[6], m[7]);
+            M.c2 = vec4(m[8], m[9], m[10], m[11]);
+            M.c3 = vec4(m[12], m[13], m[14], m[15]);
+            TlasInstance ti;
+            tlas_inverse(M, ti.m);
+            const vec3 o = tlas_point(ti, wo), d = tlas_vector(ti, wd);
+            const RayShear sh = make_shear(d);
+            const uint32_t vo = inst_ranges[3 * k], fo = inst_ranges[3 * k + 1], fc = inst_ranges[3 * k + 2];
+            for (uint32_t f = 0; f < fc; f++) {
+                const uint32_t* ix = faces + 3 * (size_t)(fo + f);
+                const float* p0 = vertices + 3 * (size_t)(vo + ix[0]);
+                const float* p1 = vertices + 3 * (size_t)(vo + ix[1]);
+                const float* p2 = vertices + 3 * (size_t)(vo + ix[2]);
+                float tt, u, v, det;
+                if (intersect_triangle_uvd(o, sh, vec3(p0[0], p0[1], p0[2]), vec3(p1[0], p1[1], p1[2]),
+                                           vec3(p2[0], p2[1], p2[2]), r[3], r[7], tt, u, v, det) &&
+                    hit_before(tt, (uint32_t)k, f, bt, bi, bp)) {
+                    bt = tt;
+                    bi = (uint32_t)k;
+                    bp = f;
+                    bu = u;
+                    bv = v;
+                    bd = det;
+                    found = true;
+                    if (any) break;
+                }
+            }
+        }
+        t[i] = found ? bt : -1.0f;
+        inst[i] = bi;
+        prim[i] = bp;
+        bary[2 * i] = bu / bd;
+        bary[2 * i + 1] = bv / bd;
     }
 }
 

@@ -68,12 +68,17 @@ def test_autotune_times_candidates_and_caches(device, jobs, tmp_path):
     try:
         b = api.Batch(device, [(s, w, h, spp, depth, 0) for s, w, h, spp, depth in jobs])
         cfg, tried, hit = b.autotune()
-        assert not hit and len(tried) == 8 and all(t[2] > 0 for t in tried)
-        assert cfg.queue_paths in [t[0] for t in tried]
+        # stage 1: 4 queue depths x sort off/on; stage 2: trace grids and sort bins from the best
+        assert not hit and len(tried) >= 8 and all(t[4] > 0 for t in tried)
+        assert [(t[0], t[1]) for t in tried[:8]] == [(q, m) for q in (1 << 21, 1 << 22, 1 << 23, 1 << 24) for m in (0, 1)]
+        knobs = lambda c: (c.queue_paths, c.material_sort, c.trace_blocks_per_cu, c.sort_key_shift)
+        assert knobs(cfg) in [tuple(t[:4]) for t in tried]
+        assert max(tried, key=lambda t: t[4])[:4] == knobs(cfg)
         cfg2, tried2, hit2 = b.autotune()
-        assert hit2 and not tried2 and (cfg2.queue_paths, cfg2.material_sort) == (cfg.queue_paths, cfg.material_sort)
+        assert hit2 and not tried2 and knobs(cfg2) == knobs(cfg)
         # the calibration renders leave no trace in the images
-        st = b.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort)
+        st = b.render(queue_paths=cfg.queue_paths, material_sort=cfg.material_sort,
+                      trace_blocks_per_cu=cfg.trace_blocks_per_cu, sort_key_shift=cfg.sort_key_shift)
         s, w, h, spp, depth = jobs[0]
         ref, _ = solo(s, w, h, spp, depth)
         assert np.abs(b.image(0) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-6)

@@ -221,12 +221,15 @@ def graphs(device):
     return desc, scene, orc
 
 
-@pytest.mark.parametrize("sort", [0, 1])
-def test_render_graphs_per_sample(graphs, sort):
+@pytest.mark.parametrize("sort,grid,shift", [(0, 0, 0), (1, 0, 0), (1, 1, 3)])
+def test_render_graphs_per_sample(graphs, sort, grid, shift):
+    """Per sample against the oracle, unsorted, sorted by material, and with
+    the tuner's other knobs: one persistent trace workgroup per CU and 8
+    materials per sort bin."""
     desc, scene, orc = graphs
     W, H = 48, 48
     r = api.Renderer(scene, W, H, max_depth=8)
-    r.set_uniforms(material_sort=sort)
+    r.set_uniforms(material_sort=sort, trace_blocks_per_cu=grid, sort_key_shift=shift)
     r.set_uniforms(accumulate=0)  # each call's image holds that call's samples
     total = np.zeros((H, W, 4), np.float32)
     for frame in range(2):

@@ -1,0 +1,107 @@
+"""The two-level acceleration structure on the GPU (rm_scene_build_accel_two_level):
+instances of shared shapes, hits bit for bit against the oracle's exhaustive
+two-level search, per-sample renders, and agreement with the flat form."""
+import time
+
+import numpy as np
+import pytest
+
+import raymond_amd.api as api
+import raymond_amd.scenes as scenes
+from oracle import binding
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+W, H = 64, 48
+T = parity.ORACLE_THREADS
+
+
+@pytest.fixture(scope="module")
+def inst(device):
+    desc = scenes.instanced_field(W, H, grid=6, shapes=3, level=2)
+    desc.envmap_resolution = 64
+    scene = scenes.build_device_scene(device, desc, two_level=True)
+    orc = binding.OracleScene(desc)
+    orc.build_envmap(threads=T)
+    orc.set_two_level(True)
+    return desc, scene, orc
+
+
+def test_two_level_stores_each_shape_once(inst):
+    desc, scene, orc = inst
+    st = scene.stats()
+    assert st.two_level == 1
+    per_shape = scenes.icosphere(2)[1].shape[0]
+    assert st.stored_triangles == 3 * per_shape + 2
+    assert st.triangles == 36 * per_shape + 2
+
+
+def test_two_level_hits_match_oracle(inst):
+    desc, scene, orc = inst
+    cam = parity.camera_rays(desc, W, H, seed=41)
+    g = scene.trace_closest(cam)
+    r = orc.trace_closest(cam, threads=T)
+    parity.assert_hits_equal(g, r, cam)
+    bounce = parity.bounce_rays(cam, r[0], seed=42, count=8192)
+    parity.assert_hits_equal(scene.trace_closest(bounce), orc.trace_closest(bounce, threads=T), bounce)
+    rng = np.random.default_rng(43)
+    shadow = parity.bounce_rays(cam, r[0], seed=44, count=8192)
+    shadow[:, 7] = rng.uniform(0.05, 3.0, shadow.shape[0])
+    occ = scene.trace_any(shadow)
+    assert np.array_equal(occ, orc.trace_any(shadow, threads=T))
+    assert 0.05 < np.mean(occ) < 0.95
+    assert np.mean(r[1] < 36) > 0.3  # instances, not only the ground
+
+
+def test_two_level_render_per_sample(inst):
+    desc, scene, orc = inst
+    r = api.Renderer(scene, W, H, max_depth=6)
+    r.set_uniforms(accumulate=0)
+    for frame in range(2):
+        st = r.execute(1)
+        ref, res = orc.render(W, H, 6, frames=1, frame_begin=frame, threads=T)
+        assert [st.rays[d] for d in range(6)] == [res.rays[d] for d in range(6)]
+        assert [st.shadow_rays[d] for d in range(5)] == [res.shadow_rays[d] for d in range(5)]
+        worst = parity.assert_radiance_close(r.image(), ref, what="two-level frame %d" % frame)
+        print("two-level frame %d: worst relative error %.3g" % (frame, worst), flush=True)
+
+
+def test_two_level_agrees_with_flat(device, inst):
+    """The same rays through the flat form: the same triangles are hit (up to
+    rays grazing an edge), at distances within 1e-5 relative (the object-space
+    map rounds differently from the world-space vertices)."""
+    desc, scene, orc = inst
+    flat = scenes.build_device_scene(device, desc, envmap=False)
+    cam = parity.camera_rays(desc, W, H, seed=45)
+    a = scene.trace_closest(cam)
+    b = flat.trace_closest(cam)
+    both = (a[0] >= 0) & (b[0] >= 0)
+    assert np.mean((a[0] >= 0) == (b[0] >= 0)) > 0.999
+    assert np.mean((a[1] == b[1]) & (a[2] == b[2]) | ~both) > 0.999
+    assert np.all(np.abs(a[0][both] - b[0][both]) <= 1e-5 * b[0][both])
+
+
+def test_two_level_throughput(device):
+    """A 16 x 16 field of 4 shapes (level 5, 5.2M triangles placed, 82k
+    stored) rendered flat and two-level: the rates are printed, the device
+    memory of the two-level form is asserted to be under half the flat's."""
+    desc = scenes.instanced_field(640, 360, grid=16, shapes=4, level=5)
+    desc.envmap_resolution = 256
+    out, mem = {}, {}
+    for two in (False, True):
+        t0 = time.time()
+        scene = scenes.build_device_scene(device, desc, two_level=two)
+        setup = time.time() - t0
+        st = scene.stats()
+        r = api.Renderer(scene, 640, 360, max_depth=8, samples_per_launch=4)
+        r.execute(4)
+        s = r.execute(16)
+        out[two] = s.total_rays / (s.ms_total * 1e-3) / 1e6
+        mem[two] = st.device_bytes
+        print("%s: %d triangles placed, %d stored, %.1f MB on the device, setup %.1f s, %.1f Mrays/s"
+              % ("two-level" if two else "flat", st.triangles, st.stored_triangles, st.device_bytes / 1e6, setup,
+                 out[two]), flush=True)
+    assert out[True] > 0 and out[False] > 0
+    # both hold the same traversal spill rows (52 x 8 B per persistent lane)
+    assert 0 < mem[True] < mem[False] / 2
