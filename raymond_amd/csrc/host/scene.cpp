@@ -4,6 +4,8 @@
 // Restates the device-facing parts of host/scene/ShapeBuilder.swift:251-340,
 // EntityBuilder.swift:79-173, LightBuilder.swift:83-281 and
 // host/lights/distribution.m:7-39.
+#include <array>
+#include <thread>
 #include <tuple>
 #include <map>
 #include <algorithm>
@@ -301,7 +303,6 @@ TwoLevelBuild build_two_level(const std::vector<float>& vertices, const std::vec
     const size_t ni = insts.size();
     struct Shape {
         uint32_t vo, fo, fc, root;
-        double lo[3], hi[3];
     };
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t> shape_id;
     std::vector<Shape> shapes;
@@ -312,27 +313,19 @@ TwoLevelBuild build_two_level(const std::vector<float>& vertices, const std::vec
         auto it = shape_id.find(key);
         if (it == shape_id.end()) {
             it = shape_id.emplace(key, (uint32_t)shapes.size()).first;
-            shapes.push_back(Shape{id.vertex_offset, id.face_offset, id.face_count, 0, {0, 0, 0}, {0, 0, 0}});
+            shapes.push_back(Shape{id.vertex_offset, id.face_offset, id.face_count, 0});
         }
         inst_shape[k] = it->second;
     }
     int depth = 0;
     for (Shape& sh : shapes) {
         std::vector<BuildTri> bt(sh.fc);
-        for (int a = 0; a < 3; a++) {
-            sh.lo[a] = INFINITY;
-            sh.hi[a] = -INFINITY;
-        }
         for (uint32_t f = 0; f < sh.fc; f++) {
             const uint32_t* ix = &indices[3 * ((size_t)sh.fo + f)];
             vec3* v[3] = {&bt[f].v0, &bt[f].v1, &bt[f].v2};
             for (int j = 0; j < 3; j++) {
                 const float* p = &vertices[3 * ((size_t)sh.vo + ix[j])];
                 *v[j] = vec3(p[0], p[1], p[2]);
-                for (int a = 0; a < 3; a++) {
-                    sh.lo[a] = std::min(sh.lo[a], (double)p[a]);
-                    sh.hi[a] = std::max(sh.hi[a], (double)p[a]);
-                }
             }
             bt[f].inst = 0;
             bt[f].prim = f;
@@ -350,27 +343,51 @@ TwoLevelBuild build_two_level(const std::vector<float>& vertices, const std::vec
         depth = std::max(depth, b.depth);
         out.stored += sh.fc;
     }
-    // top level: each instance with geometry as the box (lo, hi, hi)
-    std::vector<BuildTri> boxes;
+    // top level: each instance with geometry as the box (lo, hi, hi).  The
+    // box bounds the shape's vertices mapped to the world in double (tighter
+    // than the mapped object box: a turned shape's box grows by up to sqrt 2
+    // per axis), then padded and rounded outward.
+    std::vector<std::vector<uint32_t>> shape_verts(shapes.size());
+    for (size_t si = 0; si < shapes.size(); si++) {
+        const Shape& sh = shapes[si];
+        std::vector<uint32_t>& vs = shape_verts[si];
+        vs.assign(indices.begin() + 3 * (size_t)sh.fo, indices.begin() + 3 * ((size_t)sh.fo + sh.fc));
+        std::sort(vs.begin(), vs.end());
+        vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+    }
     std::vector<mat4> xf(ni);
+    std::vector<std::array<double, 6>> wbox(ni);
+    for (size_t k = 0; k < ni; k++) xf[k] = mat4_from(insts[k].transform);
+    const int nthreads = std::max(1, std::min<int>(threads > 0 ? threads : (int)std::thread::hardware_concurrency(), 64));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++) {
+        pool.emplace_back([&, t] {
+            for (size_t k = (size_t)t; k < ni; k += (size_t)nthreads) {
+                const Shape& sh = shapes[inst_shape[k]];
+                const mat4& M = xf[k];
+                const double m[3][4] = {{M.c0.x, M.c1.x, M.c2.x, M.c3.x}, {M.c0.y, M.c1.y, M.c2.y, M.c3.y},
+                                        {M.c0.z, M.c1.z, M.c2.z, M.c3.z}};
+                std::array<double, 6> b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+                for (const uint32_t v : shape_verts[inst_shape[k]]) {
+                    const float* p = &vertices[3 * ((size_t)sh.vo + v)];
+                    for (int a = 0; a < 3; a++) {
+                        const double w = m[a][0] * p[0] + m[a][1] * p[1] + m[a][2] * p[2] + m[a][3];
+                        b[a] = std::min(b[a], w);
+                        b[3 + a] = std::max(b[3 + a], w);
+                    }
+                }
+                wbox[k] = b;
+            }
+        });
+    }
+    for (std::thread& th : pool) th.join();
+    std::vector<BuildTri> boxes;
     for (size_t k = 0; k < ni; k++) {
         const Shape& sh = shapes[inst_shape[k]];
         out.total += sh.fc;
-        xf[k] = mat4_from(insts[k].transform);
         if (sh.fc == 0) continue;
-        const mat4& M = xf[k];
-        const double m[3][4] = {{M.c0.x, M.c1.x, M.c2.x, M.c3.x}, {M.c0.y, M.c1.y, M.c2.y, M.c3.y},
-                                {M.c0.z, M.c1.z, M.c2.z, M.c3.z}};
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int c = 0; c < 8; c++) {
-            const double p[3] = {(c & 1) ? sh.hi[0] : sh.lo[0], (c & 2) ? sh.hi[1] : sh.lo[1],
-                                 (c & 4) ? sh.hi[2] : sh.lo[2]};
-            for (int a = 0; a < 3; a++) {
-                const double w = m[a][0] * p[0] + m[a][1] * p[1] + m[a][2] * p[2] + m[a][3];
-                lo[a] = std::min(lo[a], w);
-                hi[a] = std::max(hi[a], w);
-            }
-        }
+        const double lo[3] = {wbox[k][0], wbox[k][1], wbox[k][2]};
+        const double hi[3] = {wbox[k][3], wbox[k][4], wbox[k][5]};
         double mag = 0.0;
         for (int a = 0; a < 3; a++) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
         const double pad = 1e-5 * mag + 1e-30;

@@ -65,6 +65,22 @@ struct StackRef {
     }
 };
 
+// Postponed leaf sets a lane may hold while it takes nodes (1 or 2), and the
+// share of a wave's active lanes held up by their triangles (1/kHeldShare)
+// that starts a triangle phase.  On the 5.2M-triangle sphere field
+// (Mrays/s, against 2944-2976 for one slot at a quarter): two slots at 1/4
+// 3035, 1/5 3049, 1/6 3037-3071, 1/8 3051, 1/12 3018, 1/16 2996; at 1/3
+// 2935 and 1/2 2773.  Two slots visit more nodes per ray (16.1 against
+// 15.3: closer hits arrive later) but keep more lanes busy.
+#ifndef RM_LEAF_SLOTS
+#define RM_LEAF_SLOTS 2
+#endif
+constexpr int kLeafSlots = RM_LEAF_SLOTS;
+#ifndef RM_HELD_SHARE
+#define RM_HELD_SHARE 6
+#endif
+constexpr int kHeldShare = RM_HELD_SHARE;
+
 // Traversal state of one lane's ray.  It persists across steps, so a lane
 // whose ray has finished takes a new one while the rest of the wave goes on.
 struct RayState {
@@ -77,6 +93,7 @@ struct RayState {
     bool found;
     uint2 cur;  // node group: child base, (remaining child hits << 24) | internal mask
     uint2 tg;   // postponed leaf triangles: base, mask (top level of a two-level tree: instances)
+    uint2 tg2;  // a second postponed leaf set (kLeafSlots == 2)
     int sp;
     uint32_t n_nodes, n_tris;
     // two-level trees: the world ray, the instance entered, in the top level
@@ -110,14 +127,11 @@ struct RayState {
         bdet = 1.0f;
         found = false;
         tg = make_uint2(0u, 0u);
+        tg2 = make_uint2(0u, 0u);
         n_nodes = 0;
         n_tris = 0;
     }
 
-    // One node (after a stack pop when the current group is used up, so
-    // that no step is a bare pop), or the postponed triangles of the leaves
-    // hit so far.  Returns true when the traversal has ended (ANY: also at
-    // the first hit in (tmin, tmax)).
     // Two-level trees: enters the next instance of the top-level leaf set in
     // tg.  The top-level group (if it has hits left) and a marker entry with
     // the set's other instances go on the stack; popping the marker returns
@@ -150,25 +164,38 @@ struct RayState {
         tg = make_uint2(0u, 0u);
     }
 
+    // One node (after a stack pop when the current group is used up, so
+    // that no step is a bare pop), or the postponed triangles of the leaves
+    // hit so far.  Returns true when the traversal has ended (ANY: also at
+    // the first hit in (tmin, tmax)).
     template <bool ANY, bool STATS, bool TWO = false>
     __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris,
                          const TlasInstance* __restrict__ insts, const StackRef& stk) {
         // A top-level leaf set holds instances, entered right away (one per
         // step), so that tg only ever holds triangles below.
         if (TWO && top && tg.y) enter(insts, stk);
-        // Leaf triangles wait until at least a quarter of the wave's active
-        // lanes have some (or no lane can take a node), then those lanes test
-        // a few each together instead of each node step running a loop most
-        // lanes sit out.  (A quarter measured best against a half, an eighth
-        // and a sixteenth on the 5.2M-triangle sphere field.)
+        // Leaf triangles wait until 1/kHeldShare of the wave's active lanes
+        // are held up by them (or no lane can take a node), then those lanes
+        // test a few each together instead of each node step running a loop
+        // most lanes sit out.  With a second leaf slot, a lane with one
+        // pending leaf set goes on taking nodes until a second one arrives.
+        // Two-level trees keep one slot: the pending triangles must be tested
+        // before the lane leaves their instance.
+        constexpr bool kTwoSlots = kLeafSlots > 1 && !TWO;
         const bool has_tris = tg.y != 0u;
         const bool node_work = (cur.y & 0xff000000u) || sp > 0;
-        const unsigned long long want = __ballot(has_tris);
-        if (__popcll(want) * 4 >= __popcll(__ballot(true)) || __ballot(node_work && !has_tris) == 0ull) {
+        const bool can_node = node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
+        const unsigned long long held = __ballot(has_tris && !can_node);
+        if (__popcll(held) * kHeldShare >= __popcll(__ballot(true)) || __ballot(can_node) == 0ull) {
             if (!has_tris) return !node_work;
-            return test_tris<ANY, STATS, TWO>(tris) || (!node_work && tg.y == 0u);
+            const bool ended = test_tris<ANY, STATS, TWO>(tris);
+            if (kTwoSlots && tg.y == 0u) {
+                tg = tg2;
+                tg2 = make_uint2(0u, 0u);
+            }
+            return ended || (!node_work && tg.y == 0u);
         }
-        if (has_tris) return false;
+        if (!can_node) return false;
         if (!(cur.y & 0xff000000u)) {
             if (sp == 0) return true;
             sp--;
@@ -266,16 +293,21 @@ struct RayState {
         // (14.1 against 15.3 per ray on the 5.2M-triangle sphere field) but
         // cost more per node step than it saved (2921 against 2948 Mrays/s).
         cur = make_uint2(q1.x, (ihits << 24) | imask);
-        tg = make_uint2(q1.y, thits);
+        if (kTwoSlots && tg.y != 0u) tg2 = make_uint2(q1.y, thits);
+        else tg = make_uint2(q1.y, thits);
         return false;
     }
 
     // Tests up to kTrisPerPhase postponed triangles; true when an any-hit
     // query found a hit.  Bounding the phase frees a lane with few triangles
     // left for node steps while others still have some.  On the 5.2M-triangle
-    // sphere field (Mrays/s): 1 per phase 2841, 2 per phase 2909, 3 2891,
-    // 4 2870; emptying every lane's list per phase was slower than 1.
-    static constexpr int kTrisPerPhase = 2;
+    // sphere field (Mrays/s), one leaf slot: 1 per phase 2841, 2 per phase
+    // 2909, 3 2891, 4 2870, emptying every lane's list per phase slower than
+    // 1; two slots: 1 2899, 2 3033-3071, 3 3052.
+#ifndef RM_TRIS_PER_PHASE
+#define RM_TRIS_PER_PHASE 2
+#endif
+    static constexpr int kTrisPerPhase = RM_TRIS_PER_PHASE;
     template <bool ANY, bool STATS, bool TWO>
     __device__ bool test_tris(const BvhTri* __restrict__ tris) {
         for (int k = 0; k < kTrisPerPhase && tg.y; k++) {

@@ -3,7 +3,8 @@
 is chosen when raymond_amd.api is imported.  With one LDS row every stack
 entry below the first lives in the global spill rows, so this exercises
 their indexing (row stride = the persistent grid's thread count) and the
-sizing of the spill buffer for scenes, raw ray batches, renders and batches.
+sizing of the spill buffer for scenes, raw ray batches, renders, batches
+and two-level scenes.
 
 Run by tests/test_full_size_gpu.py as `python -m tests.spill_check`; prints
 one JSON line.
@@ -52,6 +53,25 @@ def main():
         worst = parity.assert_radiance_close(r.image(), ref, what=name)
         out[name] = dict(bvh_depth=int(depth), rays=int(rays.shape[0]), worst_rel=worst)
         kept.append((name, desc, scene, orc))
+    # two-level: the marker entries and both trees' groups in the spill rows
+    inst = scenes.instanced_field(48, 36, grid=4, shapes=2, level=2)
+    inst.envmap_resolution = 64
+    scene = scenes.build_device_scene(dev, inst, two_level=True)
+    orc = binding.OracleScene(inst)
+    orc.build_envmap(threads=parity.ORACLE_THREADS)
+    orc.set_two_level(True)
+    cam = parity.camera_rays(inst, 48, 36, seed=5)
+    ref = orc.trace_closest(cam, threads=parity.ORACLE_THREADS)
+    parity.assert_hits_equal(scene.trace_closest(cam), ref)
+    shadow = parity.bounce_rays(cam, ref[0], seed=6, count=4096, tmax=2.0)
+    assert np.array_equal(scene.trace_any(shadow), orc.trace_any(shadow, threads=parity.ORACLE_THREADS))
+    r = api.Renderer(scene, 48, 36, max_depth=6)
+    r.set_uniforms(accumulate=0)
+    st = r.execute(1)
+    img, res = orc.render(48, 36, 6, frames=1, threads=parity.ORACLE_THREADS)
+    assert st.total_rays == res.total_rays, ("two-level", st.total_rays, res.total_rays)
+    out["two_level"] = dict(bvh_depth=int(scene.stats().bvh_depth), rays=int(cam.shape[0] + shadow.shape[0]),
+                            worst_rel=parity.assert_radiance_close(r.image(), img, what="two-level"))
     # a batch keeps its own spill rows for the merged scene; its jobs share
     # one material library, so the second job is the plain Cornell box
     plain = scenes.cornell_box(32, 40)

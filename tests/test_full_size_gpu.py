@@ -331,6 +331,6 @@ def test_traversal_spill_rows():
     res = json.loads(p.stdout.strip().splitlines()[-1])
     # the scenes traced alone reach the spill rows (depth >= 3); the batch
     # entries carry only their image error
-    for name in ("tessellated", "spheres"):
+    for name in ("tessellated", "spheres", "two_level"):
         assert res[name]["bvh_depth"] >= 3, (name, res[name])
-    assert set(res) == {"tessellated", "spheres", "cornell"} and "batch_worst_rel" in res["cornell"]
+    assert set(res) == {"tessellated", "spheres", "two_level", "cornell"} and "batch_worst_rel" in res["cornell"]
