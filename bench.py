@@ -326,7 +326,7 @@ def main():
                     help="first copy a buffer of this many MiB on the device (calibrates HBM counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-pixels", type=int, default=0, help="oracle sample size (0 = full frames)")
-    ap.add_argument("--cpu-frames", type=int, default=16, help="oracle sample: frames (spp) of the full image")
+    ap.add_argument("--cpu-frames", type=int, default=24, help="oracle sample: frames (spp) of the full image (about 10 s on 16 threads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU (the path tracer is single-GPU); "
