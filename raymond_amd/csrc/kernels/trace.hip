@@ -338,7 +338,17 @@ struct RayState {
     }
 };
 
-// Lanes are refilled once at least this many of a wave's 64 are idle.
+// Waves per SIMD the trace kernels ask the register allocator for.  Left
+// free they take 95 and 92 VGPRs (5 waves); asking for 6 gives 80 VGPRs
+// with 48 and 36 bytes of spills per lane, which measured 2311 against
+// 3079-3097 Mrays/s on the 5.2M-triangle sphere field.
+#ifndef RM_TRACE_MIN_WAVES
+#define RM_TRACE_MIN_WAVES 1
+#endif
+constexpr int kTraceMinWaves = RM_TRACE_MIN_WAVES;
+
+// Lanes are refilled once at least this many of a wave's 64 are idle
+// (with two leaf slots, 4, 12 and 16 measured within 1% of 8).
 #ifndef RM_REFILL_LANES
 #define RM_REFILL_LANES 8
 #endif
@@ -428,7 +438,7 @@ struct RayPool {
 // clip range, continuation rays (eps, inf) (intersection.metal:201-202);
 // raw batches bring their own ranges.
 template <bool STATS, bool TWO>
-__global__ __launch_bounds__(kTraceBlock) void trace_closest_kernel(ClosestArgs a) {
+__global__ __launch_bounds__(kTraceBlock, kTraceMinWaves) void trace_closest_kernel(ClosestArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
@@ -577,7 +587,7 @@ __global__ void regenerate_kernel(RegenArgs a) {
 // (weight, 1) to its pixel (shadow.metal:6-29); otherwise the occlusion flag
 // is written to `occluded`.
 template <bool STATS, bool TWO>
-__global__ __launch_bounds__(kTraceBlock) void trace_any_kernel(AnyArgs a) {
+__global__ __launch_bounds__(kTraceBlock, kTraceMinWaves) void trace_any_kernel(AnyArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
