@@ -375,6 +375,15 @@ void rm_cache_destroy(rm_cache* c) { delete c; }
 int rm_cache_get_stats(rm_cache* c, rm_cache_stats* out) {
     RM_NULL_CHECK(c);
     RM_NULL_CHECK(out);
+    RM_GUARD_BEGIN
+    // training inside rm_render does not wait for its loss; the last step's
+    // sum is still on the device (rm_render has returned, so it is final)
+    if (c->steps > 0 && c->loss.ptr) {
+        RM_HIP(hipSetDevice(c->dev->ordinal));
+        float l = 0.0f;
+        RM_HIP(hipMemcpy(&l, c->loss.ptr, sizeof(float), hipMemcpyDeviceToHost));
+        c->last_loss = l;
+    }
     out->steps = c->steps;
     out->samples = c->samples;
     out->queries = c->queries;
@@ -382,7 +391,7 @@ int rm_cache_get_stats(rm_cache* c, rm_cache_stats* out) {
     out->last_loss = c->last_loss;
     out->ms_infer = c->ms_infer;
     out->ms_train = c->ms_train;
-    return RM_OK;
+    RM_GUARD_END
 }
 
 uint64_t rm_cache_table_floats(rm_cache* c) { return c ? (uint64_t)c->table_floats : 0; }

@@ -27,8 +27,8 @@ namespace rm {
 void cache_create(rm_device* dev, const rm_cache_desc* desc, rm_cache* c);
 RcParams cache_params(const rm_cache* c);
 // One Adam step on `batch` samples drawn from `smp` (its permutation fields
-// select them); returns the mean loss only when `read_loss` (it synchronises).
-// One Adam step on `batch` samples, on `stream` (null: the device's stream).
+// select them), on `stream` (null: the device's stream); returns the mean
+// loss only when `read_loss` (it synchronises).
 float cache_step(rm_cache* c, const RcSamples& smp, uint32_t batch, bool read_loss, hipStream_t stream = nullptr);
 void cache_read_params(rm_cache* c, float* mlp, size_t mlp_count, float* table, size_t table_count);
 void cache_write_params(rm_cache* c, const float* mlp, size_t mlp_count, const float* table, size_t table_count);

@@ -100,6 +100,7 @@ struct RayState {
     vec3 wo, wd;
     uint32_t inst;
     bool top;
+    int marker_sp;  // stack index of the instance's marker entry (-1 in the top level)
 
     __device__ void set_ray(vec3 org, vec3 d) {
         o = org;
@@ -114,6 +115,7 @@ struct RayState {
         wd = d;
         inst = 0;
         top = true;
+        marker_sp = -1;
         // a one-child pseudo node whose only child is the root
         cur = make_uint2(root, (1u << (24 + oct)) | 1u);
         sp = 0;
@@ -148,6 +150,7 @@ struct RayState {
             sp++;
         }
         stk.store(sp, make_uint2(tg.x, kMarker2l | tg.y));
+        marker_sp = sp;
         sp++;
         TlasInstance ti;
         const uint4 rows[3] = {r0, r1, r2};
@@ -179,12 +182,13 @@ struct RayState {
         // test a few each together instead of each node step running a loop
         // most lanes sit out.  With a second leaf slot, a lane with one
         // pending leaf set goes on taking nodes until a second one arrives.
-        // Two-level trees keep one slot: the pending triangles must be tested
-        // before the lane leaves their instance.
-        constexpr bool kTwoSlots = kLeafSlots > 1 && !TWO;
+        // In a two-level tree a lane tests its pending triangles before it
+        // pops its instance's marker: they belong to the instance's ray.
+        constexpr bool kTwoSlots = kLeafSlots > 1;
         const bool has_tris = tg.y != 0u;
         const bool node_work = (cur.y & 0xff000000u) || sp > 0;
-        const bool can_node = node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
+        bool can_node = node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
+        if (TWO && kTwoSlots && has_tris && !(cur.y & 0xff000000u) && sp - 1 == marker_sp) can_node = false;
         const unsigned long long held = __ballot(has_tris && !can_node);
         if (__popcll(held) * kHeldShare >= __popcll(__ballot(true)) || __ballot(can_node) == 0ull) {
             if (!has_tris) return !node_work;
@@ -205,6 +209,7 @@ struct RayState {
                 // top-level groups below the marker
                 set_ray(wo, wd);
                 top = true;
+                marker_sp = -1;
                 tg = make_uint2(e.x, e.y & ~kMarker2l);
                 cur = make_uint2(0u, 0u);
                 return false;

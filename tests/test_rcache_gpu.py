@@ -127,6 +127,8 @@ def test_render_with_cache(device, cornell_scene):
     img = r.image()
     assert np.all(np.isfinite(img))
     assert cs.queries > 0 and cs.train_records > 0 and cs.steps == 4 * 2
+    # the last in-render step's relative L2 loss, read back by the stats call
+    assert np.isfinite(cs.last_loss) and cs.last_loss > 0.0
     # paths end at their first diffuse vertex, so far fewer rays are traced
     assert st.total_rays < 0.7 * sp.total_rays
     # frozen cache: no training vertices, no steps
