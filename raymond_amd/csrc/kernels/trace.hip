@@ -71,7 +71,9 @@ struct StackRef {
 // (Mrays/s, against 2944-2976 for one slot at a quarter): two slots at 1/4
 // 3035, 1/5 3049, 1/6 3037-3071, 1/8 3051, 1/12 3018, 1/16 2996; at 1/3
 // 2935 and 1/2 2773.  Two slots visit more nodes per ray (16.1 against
-// 15.3: closer hits arrive later) but keep more lanes busy.
+// 15.3: closer hits arrive later) but keep more lanes busy.  Three slots
+// measured 2875 (at 1/6 and 1/8): 16.9 nodes per ray, and the closest-hit
+// kernel's 97 VGPRs fit 4 waves per SIMD.
 #ifndef RM_LEAF_SLOTS
 #define RM_LEAF_SLOTS 2
 #endif
