@@ -67,6 +67,22 @@ def test_two_level_render_per_sample(inst):
         print("two-level frame %d: worst relative error %.3g" % (frame, worst), flush=True)
 
 
+def test_two_level_lens_render_per_sample(inst):
+    """The lens camera over the two-level form: sensor rays through a singlet
+    with the stop at 0.4, spectral weighting on."""
+    desc, scene, orc = inst
+    lens = api.lens_desc(scenes.singlet_lens(50.0, 2.8), stop_index=3, relative_stop=0.4, aperture_blades=5,
+                         spectral=1)
+    r = api.Renderer(scene, W, H, max_depth=6)
+    r.set_uniforms(accumulate=0)
+    r.set_lens(lens)
+    st = r.execute(1)
+    ref, res = orc.render(W, H, 6, frames=1, frame_begin=0, lens=lens, threads=T)
+    assert st.total_rays == res.total_rays and 0 < st.rays[0] < W * H
+    worst = parity.assert_radiance_close(r.image(), ref, what="two-level lens")
+    print("two-level lens: %d camera rays, worst relative error %.3g" % (st.rays[0], worst), flush=True)
+
+
 def test_two_level_agrees_with_flat(device, inst):
     """The same rays through the flat form: the same triangles are hit (up to
     rays grazing an edge), at distances within 1e-5 relative (the object-space

@@ -96,3 +96,22 @@ def test_noalloc_matches_rm_trace_closest(spheres):
     finally:
         hip.sync()
         hip.free()
+
+
+def test_noalloc_two_level(device):
+    """The same entry point on a scene built two-level: it runs the two-level
+    kernels with the scene's instance records."""
+    desc = scenes.instanced_field(64, 48, grid=6, shapes=3, level=2)
+    scene = scenes.build_device_scene(device, desc, envmap=False, two_level=True)
+    assert scene.stats().two_level == 1
+    hip = Hip()
+    try:
+        cam = parity.camera_rays(desc, 64, 48, seed=65)
+        ref = scene.trace_closest(cam)
+        got = noalloc_trace(hip, scene, cam)
+        parity.assert_hits_equal(got, ref)
+        bounce = parity.bounce_rays(cam, ref[0], seed=66, count=20000)
+        parity.assert_hits_equal(noalloc_trace(hip, scene, bounce), scene.trace_closest(bounce))
+    finally:
+        hip.sync()
+        hip.free()
