@@ -337,12 +337,12 @@ def main():
     import raymond_amd.api as api
     import raymond_amd.scenes as scenes
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     if args.workload == "cache":
         return bench_cache(args)
     if args.workload == "batch":
         return bench_batch(args)
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
     W, H = args.width, args.height
     loaded = None
     if args.scene:
