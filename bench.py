@@ -141,7 +141,7 @@ def cache_vs_plain(args, dev=None):
     """Radiance cache against the plain tracer: the Cornell box at full HD,
     depth 12, roulette from bounce 3.  A converged FP64 render of the plain
     tracer is the reference; the plain tracer and the cached tracer (online
-    training on 1 path in 16, cold start) each render `cache_spp` spp, and the
+    training on 1 path in 2^cache_train_shift, cold start) each render `cache_spp` spp, and the
     RMSE against the reference is recorded at power-of-two sample counts with
     the time spent so far.  Returns the comparison as a dict, with the
     cached tracer's time to the plain tracer's final RMSE over the plain time
@@ -176,7 +176,8 @@ def cache_vs_plain(args, dev=None):
         cache = None
 
         def new_cache():
-            return api.RadianceCache(dev, seed=1, batch_size=args.cache_batch, steps_per_launch=args.cache_steps)
+            return api.RadianceCache(dev, log2_table_size=args.cache_log2, seed=1, train_shift=args.cache_train_shift,
+                                     batch_size=args.cache_batch, steps_per_launch=args.cache_steps)
 
         if cached:
             cache = new_cache()
@@ -231,9 +232,10 @@ def cache_vs_plain(args, dev=None):
     t_cache = time_to(cached["points"], target)
     return {
         "workload": "radiance cache: Cornell box %dx%d, depth %d, roulette from bounce 3, hash grid 16x2 FP16 x "
-                    "2^19, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in 16 trains, %d training steps of %d "
+                    "2^%d, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in %d trains, %d training steps of %d "
                     "samples per spp, %d spp each; reference: plain tracer, %d spp, FP64 accumulation"
-                    % (W, H, D, args.cache_steps, args.cache_batch, spp, args.ref_spp),
+                    % (W, H, D, args.cache_log2, 1 << args.cache_train_shift, args.cache_steps, args.cache_batch, spp,
+                       args.ref_spp),
         "spp": spp,
         "plain": plain,
         "cached": cached,
@@ -310,8 +312,13 @@ def main():
                          "its render resolution applies unless --width/--height are given")
     ap.add_argument("--jobs", type=int, default=64, help="batch workload: number of jobs")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache workload: spp of the FP64 reference")
-    ap.add_argument("--cache-batch", type=int, default=8192, help="cache workload: samples per training step")
+    # cache configuration of the comparison, from a sweep on this workload: table 2^14-2^19 entries per level,
+    # one path in 2^3-2^8 training, 1-4 steps of 2048-16384 samples per spp
+    ap.add_argument("--cache-batch", type=int, default=4096, help="cache workload: samples per training step")
     ap.add_argument("--cache-steps", type=int, default=2, help="cache workload: training steps per sample")
+    ap.add_argument("--cache-log2", type=int, default=15, help="cache workload: log2 of the hash-table entries per level")
+    ap.add_argument("--cache-train-shift", type=int, default=7,
+                    help="cache workload: one camera path in 2^shift trains")
     ap.add_argument("--cache-spp", type=int, default=64,
                     help="spp of each side of the cache-vs-plain comparison in the default line")
     ap.add_argument("--no-cache-compare", action="store_true",
