@@ -217,7 +217,10 @@ __device__ inline void rc_forward(const RcLdsFwd& W, const half4_t x[6], int n, 
     act.y = y;
 }
 
-// Three waves per SIMD is the most the register budget allows without spills.
+// Three waves per SIMD is the most the register budget allows without spills
+// (167 VGPRs).  Forcing 4 (128 VGPRs, 112 bytes of spills per lane) or 5
+// (96, 240 bytes) lost with the L2-resident 2^15 table: equal-time cache
+// ratio 0.601-0.611 and 0.624 against 0.589-0.591.
 // Query rows are read through the spatial order; when the grid is a multiple
 // of 8 workgroups, the tiles are cut into 8 contiguous ranges, one per
 // workgroup residue mod 8 (consecutive workgroups go to different XCDs), so
