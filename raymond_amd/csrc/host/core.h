@@ -166,7 +166,7 @@ struct rm_renderer {
     uint32_t cache_parity = 0;         // which of the two queue sets the next stream fills
     rm::DevBuf slot_buf, shadow_slot_buf, target_buf;
     rm::DevBuf slot_rad_buf[2], query_buf[2], train_buf[2];  // per stream parity
-    rm::DevBuf qsort_buf;  // lookup sort: keys, order, predictions, bin starts and cursors
+    rm::DevBuf qsort_buf;  // lookup and training-record sorts: keys, order, bin starts and cursors
     // Training and lookups of one stream run on their own HIP stream while
     // the next stream traces; done[p] marks the end of the cache work that
     // reads the queue set of parity p.

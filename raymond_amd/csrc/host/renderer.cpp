@@ -191,6 +191,22 @@ static void join_cache_work(rm_renderer* r) {
 // Buffers of the radiance cache for a stream of `samples` samples per pixel,
 // filling the queue set of parity `par`.  They are sized for the longest
 // stream at once, so that growing streams allocate nothing on the way.
+// The spatial sorts of a stream's lookups and of its training records share
+// qsort_buf (the records are sorted and consumed before the lookups): keys
+// and order (one word per query each), then the bin starts and cursors.
+static size_t qsort_bytes(size_t queries) { return queries * 2 * 4 + 2 * kRcSortBins * 4; }
+static RcSortArgs qsort_layout(rm_renderer* r) {
+    uint32_t* qs = r->qsort_buf.as<uint32_t>();
+    const size_t qc = r->query_capacity;
+    RM_REQUIRE(r->qsort_buf.bytes >= qsort_bytes(qc), "lookup sort buffer smaller than its layout");
+    RcSortArgs so{};
+    so.keys = qs;
+    so.order = qs + qc;
+    so.hist = qs + 2 * qc;
+    so.cursor = so.hist + kRcSortBins;
+    return so;
+}
+
 static void prepare_cache(rm_renderer* r, uint32_t samples, uint32_t par) {
     rm_cache* c = r->cache;
     CacheView& cv = r->cv;
@@ -205,8 +221,7 @@ static void prepare_cache(rm_renderer* r, uint32_t samples, uint32_t par) {
     if (queries > r->query_capacity) {
         r->query_capacity = queries;
         for (DevBuf& b : r->query_buf) b.alloc(queries * kRcQueryFloats * 4);
-        // keys, order (u32 each), predictions (4 floats), bin starts + cursors
-        r->qsort_buf.alloc(queries * 6 * 4 + 2 * kRcSortBins * 4);
+        r->qsort_buf.alloc(qsort_bytes(queries));
         RM_HIP(hipMemsetAsync(r->qsort_buf.ptr, 0, r->qsort_buf.bytes, r->scene->dev->stream));
     }
     if (train && slots > r->slot_capacity) {
@@ -555,17 +570,11 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
                                      r->target_buf.as<float>(), cs));
             // the records in spatial order (the lookup sort's buffers are
             // free until the lookups), sampled systematically per step
-            uint32_t* qs = r->qsort_buf.as<uint32_t>();
-            const size_t qc = r->query_capacity;
-            RcSortArgs so{};
+            RcSortArgs so = qsort_layout(r);
             so.pos = r->cv.train.feat;
             so.pos_elem = 1;
             so.pos_comp = r->cv.train_capacity;
             so.count = records;
-            so.keys = qs;
-            so.order = qs + qc;
-            so.hist = qs + 6 * qc;
-            so.cursor = so.hist + kRcSortBins;
             RM_HIP(launch_rc_sort_queries(so, cus * 4, cs));
             RcSamples smp{};
             smp.feat = r->cv.train.feat;
@@ -589,32 +598,24 @@ static void render_stream(rm_renderer* r, uint32_t samples, rm_render_stats* sta
             }
         }
         // the lookups of the whole stream in one batch (counts by value: the
-        // next stream clears the counters): sorted by position, predicted,
-        // then added to the image in queue order
+        // next stream clears the counters): sorted by position, then
+        // predicted and added to their pixels in that order
         const uint32_t nq = std::min<uint32_t>(cache_ctr[0], r->cv.query_capacity);
         hipEvent_t ei = tm.begin();
         if (nq > 0) {
-            const size_t qc = r->query_capacity;
-            uint32_t* qs = r->qsort_buf.as<uint32_t>();
-            RcSortArgs so{};
+            RcSortArgs so = qsort_layout(r);
             so.pos = r->cv.queries.rows + 4;
             so.pos_elem = kRcQueryFloats;
             so.pos_comp = 1;
             so.count = nq;
-            so.keys = qs;
-            so.order = qs + qc;
-            float* pred = reinterpret_cast<float*>(qs + 2 * qc);
-            so.hist = qs + 6 * qc;
-            so.cursor = so.hist + kRcSortBins;
             RM_HIP(launch_rc_sort_queries(so, cus * 4, cs));
             RcInferArgs ia{};
             ia.p = cache_params(r->cache);
             ia.rows = r->cv.queries.rows;
             ia.order = so.order;
             ia.count = nq;
-            ia.out = pred;
+            ia.image = image;
             RM_HIP(launch_rc_infer(ia, cus * 8, cs));
-            RM_HIP(launch_rc_resolve(ia.rows, pred, nq, image, cs));
         }
         tm.end(ei, t_infer);
         if (overlap) {

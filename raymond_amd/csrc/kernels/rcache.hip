@@ -265,7 +265,16 @@ __global__ __launch_bounds__(64 * kInferWaves, 3) void rc_infer_kernel(RcInferAr
         RcAct act;
         rc_forward(W, x, n, hi, act);
         if (hi == 0 && valid) {
-            if (a.rows) {
+            if (a.rows && a.image) {
+                // the lookup's contribution straight into its pixel: throughput
+                // x prediction, and one sample in alpha
+                const row4 r0 = reinterpret_cast<const row4*>(a.rows + (size_t)row * kRcQueryFloats)[0];
+                float* px = a.image + 4u * (size_t)__builtin_bit_cast(uint32_t, r0[0]);
+                atomicAdd(px + 0, r0[1] * max_(act.y[0], 0.0f));
+                atomicAdd(px + 1, r0[2] * max_(act.y[1], 0.0f));
+                atomicAdd(px + 2, r0[3] * max_(act.y[2], 0.0f));
+                atomicAdd(px + 3, 1.0f);
+            } else if (a.rows) {
                 row4 v;
                 v[0] = max_(act.y[0], 0.0f);
                 v[1] = max_(act.y[1], 0.0f);
@@ -364,19 +373,6 @@ __global__ void rc_qclear_kernel(RcSortArgs a) {
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < kRcSortBins; b += gridDim.x * blockDim.x) {
         a.hist[b] = 0u;
         a.cursor[b] = 0u;
-    }
-}
-
-__global__ void rc_resolve_kernel(const float* rows, const float* pred, uint32_t count, float* image) {
-    typedef float row4 __attribute__((ext_vector_type(4)));
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const row4 r = reinterpret_cast<const row4*>(rows)[(size_t)i * (kRcQueryFloats / 4)];
-        const row4 p = reinterpret_cast<const row4*>(pred)[i];
-        float* px = image + 4u * (size_t)__builtin_bit_cast(uint32_t, r[0]);
-        atomicAdd(px + 0, r[1] * p[0]);
-        atomicAdd(px + 1, r[2] * p[1]);
-        atomicAdd(px + 2, r[3] * p[2]);
-        atomicAdd(px + 3, 1.0f);
     }
 }
 
@@ -710,10 +706,6 @@ hipError_t launch_rc_sort_queries(const RcSortArgs& a, int blocks, hipStream_t s
     hipLaunchKernelGGL(rc_qscan_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(rc_qscatter_kernel, dim3(blocks), dim3(kRcSortBlock), 0, s, a);
     hipLaunchKernelGGL(rc_qclear_kernel, dim3(16), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-hipError_t launch_rc_resolve(const float* rows, const float* pred, uint32_t count, float* image, hipStream_t s) {
-    hipLaunchKernelGGL(rc_resolve_kernel, dim3(2048), dim3(256), 0, s, rows, pred, count, image);
     return hipGetLastError();
 }
 hipError_t launch_rc_train(const RcParams& p, const RcSamples& smp, uint32_t batch, const RcGrads& g, float scale,

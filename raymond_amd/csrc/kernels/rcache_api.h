@@ -47,8 +47,9 @@ struct RcSamples {
 };
 
 // Lookups: either query rows read through a spatial order (the renderer's
-// path), each writing its prediction to pred[row], or feature-major raw
-// batches writing 3 x count outputs.
+// path), each adding throughput x prediction to its pixel in `image` (plus
+// one sample in alpha) or, without an image, writing its prediction to
+// out[row]; or feature-major raw batches writing 3 x count outputs.
 struct RcInferArgs {
     RcParams p;
     const float* rows;         // kRcQueryFloats per query, or null for a raw batch
@@ -57,6 +58,7 @@ struct RcInferArgs {
     uint32_t count;
     uint32_t stride;           // raw batch: feature-array stride
     float* out;                // rows: 4 floats per row (prediction, 0); raw: 3 x count
+    float* image;              // rows: add throughput x prediction to the row's pixel here instead of out
 };
 
 // Spatial order of the lookups: a counting sort by the Morton code of the
@@ -93,9 +95,7 @@ hipError_t launch_rc_infer(const RcInferArgs& a, int blocks, hipStream_t s);
 // keys + histogram, exclusive scan, scatter (hist and cursor must start at 0;
 // the scan leaves cursor at 0 for the scatter, and the scatter clears hist).
 hipError_t launch_rc_sort_queries(const RcSortArgs& a, int blocks, hipStream_t s);
-// image[pixel] += throughput x pred, in row order, plus one sample in alpha.
-hipError_t launch_rc_resolve(const float* rows, const float* pred, uint32_t count, float* image, hipStream_t s);
-// Fused forward + backward over `batch` samples (32 per wave, 4 waves per
+// Fused forward + backward over `batch` samples (32 per wave, two waves per
 // workgroup): table gradients by atomic add, MLP gradients as one partial per
 // wave.  `scale` multiplies every gradient (1 / batch for a mean loss).
 hipError_t launch_rc_train(const RcParams& p, const RcSamples& smp, uint32_t batch, const RcGrads& g, float scale,
