@@ -147,7 +147,12 @@ __device__ inline float rc_corner(const RcGrid& g, int level, const RcCell& c, i
 // lane's 8 levels, then the vertex features and the constant input.  The
 // loop over level pairs stays rolled (16 gathers in flight per lane instead
 // of 64 keeps the register count, and with it the occupancy, up); its result
-// is placed by value selects, not by indexing a register array.
+// is placed by value selects, not by indexing a register array.  The
+// per-lane branch on a level's dense flag makes the compiler issue the
+// corner gathers two at a time; a branch-free form that issued each level
+// pair's 16 gathers together measured slower on the cache comparison
+// (equal-time ratio 0.590-0.595 against 0.575-0.581), its per-level
+// parameter loads and a register spill costing more than the overlap won.
 __device__ inline void rc_inputs(const RcParams& p, const float* f, int hi, half4_t x[6]) {
     const vec3 pos(f[0], f[1], f[2]);
     half4_t x0 = {}, x1 = {}, x2 = {}, x3 = {};
