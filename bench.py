@@ -22,6 +22,8 @@ delivers.
 
 `--workload graphs` runs the shader-graph workload instead: the Cornell-box geometry
 with 32 random node-graph materials, depth 8, shading sorted by material.
+`--workload instances` renders a 16 x 16 field of instances of 4 shapes (5.2M
+triangles placed, 82k stored) through the two-level acceleration structure.
 
 Prints one JSON line.  `--gpus` above 1 is rejected: the tracer runs on one GPU.
 """
@@ -50,7 +52,12 @@ def copy_probe(mib):
     dev.close()
 
 
-def workload_name(graphs, W, H, D, sort, cornell=False):
+def workload_name(graphs, W, H, D, sort, cornell=False, instances=False):
+    if instances:
+        return ("instanced field: 16x16 instances of 4 noise-displaced level-5 icospheres + ground, "
+                "5,242,882 triangles placed, %s, analytic sky, %dx%d, depth %d, plain tracer, 1 spp per step"
+                % ("flat tree over all of them" if instances == "flat" else
+                   "81,922 stored, two-level acceleration structure", W, H, D))
     if cornell:
         return ("Cornell box: 36 triangles, emitter 15, %dx%d, depth %d, roulette from bounce 3, "
                 "plain tracer, 1 spp per step" % (W, H, D))
@@ -302,11 +309,15 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=0, help="path depth (0 = the workload's: 12 spheres, 8 graphs)")
-    ap.add_argument("--workload", choices=["spheres", "cornell", "graphs", "cache", "batch"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "cornell", "graphs", "cache", "batch", "instances"],
+                    default="spheres",
                     help="spheres: 5.2M-triangle sphere field; cornell: the 36-triangle Cornell box at 512^2; "
                          "graphs: 32 random node-graph materials; "
                          "cache: learned radiance cache against the plain tracer on the Cornell box; "
-                         "batch: 64 uneven Cornell-box jobs in one bounce loop")
+                         "batch: 64 uneven Cornell-box jobs in one bounce loop; "
+                         "instances: 16x16 instances of 4 shapes through the two-level tree")
+    ap.add_argument("--flat", action="store_true",
+                    help="instances workload: build the flat tree over the placed triangles instead (for comparison)")
     ap.add_argument("--scene", default="",
                     help="render a scene file of the Blender exporter (JSON + PLY meshes) instead of a workload; "
                          "its render resolution applies unless --width/--height are given")
@@ -363,9 +374,10 @@ def main():
                                                              loaded.num_triangles), file=sys.stderr, flush=True)
     graphs = args.workload == "graphs"
     cornell = args.workload == "cornell"
+    instances = args.workload == "instances"
     if cornell and (W, H) == (1920, 1080):
         W, H = 512, 512
-    D = args.depth or (8 if graphs or cornell else 12)
+    D = args.depth or (8 if graphs or cornell or instances else 12)
     # sorting pays for itself on every workload here: on the sphere field it
     # splits misses (sky) from hits and metals from dielectrics (shading
     # 0.53 -> 0.45 ms per spp for 0.05 ms of sorting)
@@ -381,13 +393,15 @@ def main():
     elif cornell:
         desc = scenes.cornell_box(W, H)
         desc.envmap_resolution = 64  # black world
+    elif instances:
+        desc = scenes.instanced_field(W, H, grid=16, shapes=4, level=5)
     else:
         desc = scenes.sphere_field(W, H)
     dev = api.Device(0)
     # the graph scene inlines its materials (one compile, cached; faster shading)
     inline = args.inline_materials if args.inline_materials >= 0 else (1 if graphs else -1)
     noinline = None if inline < 0 else not inline
-    scene = scenes.build_device_scene(dev, desc, noinline=noinline)
+    scene = scenes.build_device_scene(dev, desc, noinline=noinline, two_level=instances and not args.flat)
     sst = scene.stats()
     setup_s = time.time() - t0
     print("setup %.1fs: %d triangles, BVH8 %d nodes depth %d (%.1fs), JIT %.1fs, envmap %.2fs" %
@@ -431,7 +445,7 @@ def main():
     # closest rays per launch
     traffic = None
     traffic_file = os.path.join(HERE, "profiles", "trace_closest_c2_traffic.json")
-    if not graphs and not cornell and os.path.exists(traffic_file):
+    if loaded is None and not graphs and not cornell and not instances and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
         # one closest-hit launch per bounce-loop iteration
@@ -457,6 +471,8 @@ def main():
             pix = None
             frames = args.cpu_frames
             sample = "%d full %dx%d frames (%d bench steps), 1 spp each, depth %d" % (frames, W, H, frames, D)
+            if instances:
+                sample += ", the oracle's flat BVH2 over the placed triangles"
         _, res = orc.render(W, H, D, frames=frames, frame_begin=10_000, pixels=pix, threads=threads, rr_depth=rr_depth)
         cpu = {"value": res.total_rays / res.seconds / 1e6, "unit": "Mrays/s", "cores": int(res.threads_used),
                "kind": "port", "sample": sample, "seconds": round(res.seconds, 2), "cpu_model": cpu_host["model"],
@@ -478,7 +494,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": workload_name(graphs, W, H, D, sort, cornell) if loaded is None else
+            "workload": workload_name(graphs, W, H, D, sort, cornell, ("flat" if args.flat else True) if instances
+                                      else False) if loaded is None else
             "exporter scene %s: %d triangles, %dx%d, depth %d, plain tracer, 1 spp per step" %
             (os.path.basename(args.scene), int(sst.triangles), W, H, D),
             "width": W, "height": H, "max_depth": D, "queue_frames": spl, "triangles": int(sst.triangles),
@@ -503,7 +520,7 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
-    if loaded is None and not graphs and not cornell and not args.no_cache_compare:
+    if loaded is None and not graphs and not cornell and not instances and not args.no_cache_compare:
         # the metric's second half on the same device: the cache against the
         # plain tracer at equal time (the sphere field's buffers are freed first)
         del r, scene

@@ -353,6 +353,13 @@ struct RayState {
 #define RM_TRACE_MIN_WAVES 1
 #endif
 constexpr int kTraceMinWaves = RM_TRACE_MIN_WAVES;
+// The two-level kernels take 103 and 101 VGPRs left free (4 waves); held to
+// 5 waves they take 96 with 24 and 20 bytes of spills per lane, which
+// measured 2331 against 2295 Mrays/s on the 1080p instanced field.
+#ifndef RM_TWO_LEVEL_MIN_WAVES
+#define RM_TWO_LEVEL_MIN_WAVES 5
+#endif
+constexpr int kTwoLevelMinWaves = RM_TWO_LEVEL_MIN_WAVES;
 
 // Lanes are refilled once at least this many of a wave's 64 are idle
 // (with two leaf slots, 4, 12 and 16 measured within 1% of 8).
@@ -445,7 +452,7 @@ struct RayPool {
 // clip range, continuation rays (eps, inf) (intersection.metal:201-202);
 // raw batches bring their own ranges.
 template <bool STATS, bool TWO>
-__global__ __launch_bounds__(kTraceBlock, kTraceMinWaves) void trace_closest_kernel(ClosestArgs a) {
+__global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWaves) void trace_closest_kernel(ClosestArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
@@ -594,7 +601,7 @@ __global__ void regenerate_kernel(RegenArgs a) {
 // (weight, 1) to its pixel (shadow.metal:6-29); otherwise the occlusion flag
 // is written to `occluded`.
 template <bool STATS, bool TWO>
-__global__ __launch_bounds__(kTraceBlock, kTraceMinWaves) void trace_any_kernel(AnyArgs a) {
+__global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWaves) void trace_any_kernel(AnyArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
