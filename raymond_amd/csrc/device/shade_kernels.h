@@ -396,7 +396,8 @@ __device__ inline __attribute__((always_inline)) void shade_queue(const ShadeArg
 // sphere field 2772 -> 2798; Cornell box 4719 -> 4827; 6 waves lose on all
 // three, e.g. 2831 -> 2783 on the sphere field), the batch kernel,
 // which carries per-job state, at 4 (3605 against 3355 Mrays/s at 5); the
-// cache kernel keeps 4 with it.
+// cache kernel keeps 4 with it (at 5 the cache comparison's equal-time ratio
+// went from 0.575-0.583 to 0.604-0.607).
 constexpr int kShadeMinWaves = 5;
 constexpr int kShadeWideMinWaves = 4;
 
