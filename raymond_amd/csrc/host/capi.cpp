@@ -379,8 +379,7 @@ int rm_cache_get_stats(rm_cache* c, rm_cache_stats* out) {
     // training inside rm_render does not wait for its loss; the last step's
     // sum is still on the device (rm_render has returned, so it is final)
     if (c->steps > 0 && c->loss.ptr) {
-        RM_HIP(hipSetDevice(c->dev->ordinal));
-        float l = 0.0f;
+        float l = 0.0f;  // a device-to-host copy finds the buffer's device by its address
         RM_HIP(hipMemcpy(&l, c->loss.ptr, sizeof(float), hipMemcpyDeviceToHost));
         c->last_loss = l;
     }
