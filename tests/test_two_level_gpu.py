@@ -83,6 +83,14 @@ def test_two_level_lens_render_per_sample(inst):
     print("two-level lens: %d camera rays, worst relative error %.3g" % (st.rays[0], worst), flush=True)
 
 
+def test_two_level_scene_rejected_by_batches(device, inst):
+    """Batches merge flat trees only; a two-level scene is an error, not a
+    silent flat render."""
+    desc, scene, orc = inst
+    with pytest.raises(api.RmError, match="flat acceleration structure"):
+        api.Batch(device, [(scene, W, H, 1, 4, 0)])
+
+
 def test_two_level_agrees_with_flat(device, inst):
     """The same rays through the flat form: the same triangles are hit (up to
     rays grazing an edge), at distances within 1e-5 relative (the object-space
