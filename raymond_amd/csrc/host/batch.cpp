@@ -67,7 +67,8 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
         RM_REQUIRE(d.scene->accel_built && d.scene->materials_built && d.scene->envmap_built,
                    "job scenes must have their acceleration structure, materials and envmap built");
         RM_REQUIRE(d.scene->material_source == src, "the jobs of a batch must share one material library");
-        RM_REQUIRE(!d.scene->two_level, "batches take scenes with the flat acceleration structure");
+        RM_REQUIRE(d.scene->two_level == jobs[0].scene->two_level,
+                   "the jobs of a batch must all have the flat or all the two-level acceleration structure");
         RM_REQUIRE(d.width > 0 && d.height > 0 && d.samples > 0, "job size and samples must be positive");
         RM_REQUIRE(d.max_depth >= 1 && d.max_depth <= 255, "job max_depth out of range");
         pixels += (uint64_t)d.width * d.height;
@@ -81,6 +82,8 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
     std::vector<ShapeLight> shapes;
     std::vector<Bvh8Node> nodes;
     std::vector<BvhTri> tris;
+    std::vector<TlasInstance> recs;  // two-level jobs: the instance records of every job's top-level tree
+    const bool two_level = jobs[0].scene->two_level;
     std::vector<uint32_t> shape_off(n);
     uint32_t image_offset = 0;
     for (uint32_t j = 0; j < n; j++) {
@@ -107,19 +110,31 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
         }
         std::vector<Bvh8Node> sn = download<Bvh8Node>(s->d_nodes, s->stats.bvh_nodes, st);
         std::vector<BvhTri> stt = download<BvhTri>(s->d_tris, s->d_tris.bytes / sizeof(BvhTri), st);
-        for (Bvh8Node& node : sn) {
+        // a two-level scene stores its shapes' trees first and the top-level
+        // tree from root2l on; the top-level leaves index instance records
+        const uint32_t rec_off = (uint32_t)recs.size();
+        for (size_t k = 0; k < sn.size(); k++) {
+            Bvh8Node& node = sn[k];
             node.child_base += node_off;
-            node.tri_base += tri_off;
+            node.tri_base += (two_level && k >= s->root2l) ? rec_off : tri_off;
         }
         for (BvhTri& t : stt) t.inst += i_off;
         nodes.insert(nodes.end(), sn.begin(), sn.end());
         tris.insert(tris.end(), stt.begin(), stt.end());
+        if (two_level) {
+            std::vector<TlasInstance> sr = download<TlasInstance>(s->d_tlas, s->d_tlas.bytes / sizeof(TlasInstance), st);
+            for (TlasInstance& rec : sr) {
+                rec.blas_root += node_off;
+                rec.inst += i_off;
+            }
+            recs.insert(recs.end(), sr.begin(), sr.end());
+        }
 
         JobInfo ji{};
         ji.image_offset = image_offset;
         ji.width = jobs[j].width;
         ji.height = jobs[j].height;
-        ji.root = node_off;
+        ji.root = node_off + (two_level ? s->root2l : 0u);
         ji.max_depth = jobs[j].max_depth;
         ji.seed = jobs[j].seed ? jobs[j].seed : 1u;
         ji.spp = jobs[j].samples;
@@ -139,6 +154,11 @@ void batch_create(rm_device* dev, const rm_job_desc* jobs, uint32_t n, rm_batch*
     m->d_shape.upload(shapes.data(), shapes.size() * sizeof(ShapeLight), st);
     m->d_nodes.upload(nodes.data(), nodes.size() * sizeof(Bvh8Node), st);
     m->d_tris.upload(tris.data(), tris.size() * sizeof(BvhTri), st);
+    if (two_level) {
+        m->d_tlas.upload(recs.data(), recs.size() * sizeof(TlasInstance), st);
+        m->two_level = true;
+        m->stats.two_level = 1;
+    }
     m->instances = instances;
     const int bpc = std::max(dev->trace_blocks_per_cu, dev->any_blocks_per_cu);
     m->spill_threads = (size_t)dev->props.multiProcessorCount * (size_t)bpc * kTraceBlock;

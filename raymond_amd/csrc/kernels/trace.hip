@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWav
             const vec3 d(q.dx[i], q.dy[i], q.dz[i]);
             uint32_t root = TWO ? a.root2l : 0u;
             float near_clip = a.camera.near_clip, far_clip = a.camera.far_clip;
-            if (!TWO && a.batch.jobs) {
+            if (a.batch.jobs) {
                 const JobInfo& job = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])];
                 root = job.root;
                 near_clip = job.camera.near_clip;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWav
             const float tmin = a.tmin_arr ? a.tmin_arr[i] : kEps;
             const float tmax = q.tmax[i];
             uint32_t root = TWO ? a.root2l : 0u;
-            if (!TWO && a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
+            if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
             s.init(root, o, d, tmin, tmax);
             active = true;
         }

@@ -83,12 +83,29 @@ def test_two_level_lens_render_per_sample(inst):
     print("two-level lens: %d camera rays, worst relative error %.3g" % (st.rays[0], worst), flush=True)
 
 
-def test_two_level_scene_rejected_by_batches(device, inst):
-    """Batches merge flat trees only; a two-level scene is an error, not a
-    silent flat render."""
+def test_two_level_batch_matches_oracle(device, inst):
+    """Two-level jobs in one batch: the merge rebases every job's shape trees,
+    top-level tree and instance records; each job's image equals the oracle's
+    render of the same samples.  A batch mixing flat and two-level jobs is an
+    error."""
     desc, scene, orc = inst
-    with pytest.raises(api.RmError, match="flat acceleration structure"):
-        api.Batch(device, [(scene, W, H, 1, 4, 0)])
+    d2 = scenes.instanced_field(48, 40, grid=4, shapes=3, level=2)
+    d2.envmap_resolution = 64
+    s2 = scenes.build_device_scene(device, d2, two_level=True)
+    o2 = binding.OracleScene(d2)
+    o2.build_envmap(threads=T)
+    o2.set_two_level(True)
+    jobs = [(scene, W, H, 2, 5, 0), (s2, 48, 40, 3, 6, 0)]
+    b = api.Batch(device, jobs)
+    st = b.render(use_graph=1)
+    assert st.samples == W * H * 2 + 48 * 40 * 3
+    for j, (o, w, h, spp, depth) in enumerate([(orc, W, H, 2, 5), (o2, 48, 40, 3, 6)]):
+        ref, _ = o.render(w, h, depth, frames=spp, threads=T)
+        worst = parity.assert_radiance_close(b.image(j), ref, what="two-level batch job %d" % j)
+        print("two-level batch job %d: worst relative error %.3g" % (j, worst), flush=True)
+    flat = scenes.build_device_scene(device, d2)
+    with pytest.raises(api.RmError, match="all have the flat or all the two-level"):
+        api.Batch(device, [(scene, W, H, 1, 4, 0), (flat, 48, 40, 1, 4, 0)])
 
 
 def test_two_level_agrees_with_flat(device, inst):
