@@ -403,7 +403,9 @@ int rm_cache_train(rm_cache* c, const float* features, const float* targets, uin
  * BVH8 entered at the job's root, each job keeps its lights and camera, and
  * the paths of all jobs share the queues, the kernels and (optionally) the
  * material sort.  The jobs' scenes must be fully built, share one material
- * library and outlive the batch. */
+ * library and outlive the batch, and either all have the flat or all the
+ * two-level acceleration structure (each job then enters at its own
+ * top-level root). */
 typedef struct rm_batch rm_batch;
 
 typedef struct rm_job_desc {
