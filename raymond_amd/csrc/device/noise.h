@@ -76,33 +76,45 @@ RM_HD float floorfrac(float x, int& i) {
 }
 
 RM_HD float fade(float t) { return ((t * t) * t) * (t * (t * 6.0f - 15.0f) + 10.0f); }
-RM_HD float negate_if(float v, int cond) { return cond ? -v : v; }
+// v with its sign flipped when bit b of h is set: the bits of
+// ((h >> b) & 1) ? -v : v, by an XOR into the sign bit instead of a compare
+// and a select (on gfx950 the select waits on the compare's VCC write).
+#ifndef RM_NOISE_XOR_SIGN
+#define RM_NOISE_XOR_SIGN 1
+#endif
+RM_HD float negate_bit(float v, uint32_t h, int b) {
+#if RM_NOISE_XOR_SIGN
+    return u2f(f2u(v) ^ ((h << (31 - b)) & 0x80000000u));
+#else
+    return ((h >> b) & 1u) ? -v : v;
+#endif
+}
 RM_HD float mixf(float a, float b, float t) { return a + (b - a) * t; }
 
 RM_HD float grad1(int hash, float x) {
     const int h = hash & 15;
     const float g = (float)(1 + (h & 7));
-    return negate_if(g, h & 8) * x;
+    return negate_bit(g, (uint32_t)h, 3) * x;
 }
 RM_HD float grad2(int hash, float x, float y) {
     const int h = hash & 7;
     const float u = h < 4 ? x : y;
     const float v = 2.0f * (h < 4 ? y : x);
-    return negate_if(u, h & 1) + negate_if(v, h & 2);
+    return negate_bit(u, (uint32_t)h, 0) + negate_bit(v, (uint32_t)h, 1);
 }
 RM_HD float grad3(int hash, float x, float y, float z) {
     const int h = hash & 15;
     const float u = h < 8 ? x : y;
     const float vt = ((h == 12) || (h == 14)) ? x : z;
     const float v = h < 4 ? y : vt;
-    return negate_if(u, h & 1) + negate_if(v, h & 2);
+    return negate_bit(u, (uint32_t)h, 0) + negate_bit(v, (uint32_t)h, 1);
 }
 RM_HD float grad4(int hash, float x, float y, float z, float w) {
     const int h = hash & 31;
     const float u = h < 24 ? x : y;
     const float v = h < 16 ? y : z;
     const float s = h < 8 ? z : w;
-    return (negate_if(u, h & 1) + negate_if(v, h & 2)) + negate_if(s, h & 4);
+    return (negate_bit(u, (uint32_t)h, 0) + negate_bit(v, (uint32_t)h, 1)) + negate_bit(s, (uint32_t)h, 2);
 }
 
 RM_HD float bi_mix(float v0, float v1, float v2, float v3, float x, float y) {

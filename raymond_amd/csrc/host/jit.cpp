@@ -80,6 +80,9 @@ bool jit_compile(const std::string& body, const std::string& options, std::vecto
                                      "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang",
                                      "-packed-fp32-ops"};
     for (auto& o : split_options(options)) opts.push_back(o);
+    // extra compiler options for experiments (part of the cache key like the rest)
+    if (const char* e = getenv("RM_JIT_OPTIONS"))
+        for (auto& o : split_options(e)) opts.push_back(o);
 
     uint64_t h = 1469598103934665603ull;
     h = fnv1a(tu.data(), tu.size(), h);

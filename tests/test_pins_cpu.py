@@ -617,8 +617,10 @@ PLANTS = [
      check_perlin),
     ("perlin corner hash with x and y transposed", "noise.h",
      "grad3(H(X + 1, Y, Z), fx - 1.0f, fy, fz)", "grad3(H(Y, X + 1, Z), fx - 1.0f, fy, fz)", check_perlin),
-    ("grad4 sign bit", "noise.h", "return (negate_if(u, h & 1) + negate_if(v, h & 2)) + negate_if(s, h & 4);",
-     "return (negate_if(u, h & 1) + negate_if(v, h & 2)) + negate_if(s, h & 8);", check_perlin),
+    ("grad4 sign bit", "noise.h",
+     "return (negate_bit(u, (uint32_t)h, 0) + negate_bit(v, (uint32_t)h, 1)) + negate_bit(s, (uint32_t)h, 2);",
+     "return (negate_bit(u, (uint32_t)h, 0) + negate_bit(v, (uint32_t)h, 1)) + negate_bit(s, (uint32_t)h, 3);",
+     check_perlin),
 ]
 
 
