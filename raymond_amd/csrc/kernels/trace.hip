@@ -176,23 +176,29 @@ struct RayState {
     template <bool ANY, bool STATS, bool TWO = false>
     __device__ bool step(const Bvh8Node* __restrict__ nodes, const BvhTri* __restrict__ tris,
                          const TlasInstance* __restrict__ insts, const StackRef& stk) {
-        // A top-level leaf set holds instances, entered right away (one per
-        // step), so that tg only ever holds triangles below.
-        if (TWO && top && tg.y) enter(insts, stk);
         // Leaf triangles wait until 1/kHeldShare of the wave's active lanes
         // are held up by them (or no lane can take a node), then those lanes
         // test a few each together instead of each node step running a loop
         // most lanes sit out.  With a second leaf slot, a lane with one
         // pending leaf set goes on taking nodes until a second one arrives.
-        // In a two-level tree a lane tests its pending triangles before it
-        // pops its instance's marker: they belong to the instance's ray.
+        // In a two-level tree a top-level leaf set holds instances: a lane
+        // with one is held like a lane with triangles and enters the next
+        // instance in the same phase as the others test triangles, so the
+        // entry code runs for many lanes at once rather than in most steps
+        // for a few.  A lane tests its pending triangles before it pops its
+        // instance's marker: they belong to the instance's ray.
         constexpr bool kTwoSlots = kLeafSlots > 1;
-        const bool has_tris = tg.y != 0u;
+        const bool to_enter = TWO && top && tg.y != 0u;
+        const bool has_tris = !to_enter && tg.y != 0u;
         const bool node_work = (cur.y & 0xff000000u) || sp > 0;
-        bool can_node = node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
+        bool can_node = !to_enter && node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
         if (TWO && kTwoSlots && has_tris && !(cur.y & 0xff000000u) && sp - 1 == marker_sp) can_node = false;
-        const unsigned long long held = __ballot(has_tris && !can_node);
+        const unsigned long long held = __ballot((has_tris || to_enter) && !can_node);
         if (__popcll(held) * kHeldShare >= __popcll(__ballot(true)) || __ballot(can_node) == 0ull) {
+            if (to_enter) {
+                enter(insts, stk);
+                return false;
+            }
             if (!has_tris) return !node_work;
             const bool ended = test_tris<ANY, STATS, TWO>(tris);
             if (kTwoSlots && tg.y == 0u) {
