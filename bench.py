@@ -183,8 +183,9 @@ def cache_vs_plain(args, dev=None):
         cache = None
 
         def new_cache():
-            return api.RadianceCache(dev, log2_table_size=args.cache_log2, seed=1, train_shift=args.cache_train_shift,
-                                     batch_size=args.cache_batch, steps_per_launch=args.cache_steps)
+            return api.RadianceCache(dev, log2_table_size=args.cache_log2, seed=1, learning_rate=args.cache_lr,
+                                     train_shift=args.cache_train_shift, batch_size=args.cache_batch,
+                                     steps_per_launch=args.cache_steps)
 
         if cached:
             cache = new_cache()
@@ -239,10 +240,10 @@ def cache_vs_plain(args, dev=None):
     t_cache = time_to(cached["points"], target)
     return {
         "workload": "radiance cache: Cornell box %dx%d, depth %d, roulette from bounce 3, hash grid 16x2 FP16 x "
-                    "2^%d, MLP 48-64-64-3 FP16 on MFMA, Adam 1e-2, 1 path in %d trains, %d training steps of %d "
+                    "2^%d, MLP 48-64-64-3 FP16 on MFMA, Adam %g, 1 path in %d trains, %d training steps of %d "
                     "samples per spp, %d spp each; reference: plain tracer, %d spp, FP64 accumulation"
-                    % (W, H, D, args.cache_log2, 1 << args.cache_train_shift, args.cache_steps, args.cache_batch, spp,
-                       args.ref_spp),
+                    % (W, H, D, args.cache_log2, args.cache_lr, 1 << args.cache_train_shift, args.cache_steps,
+                       args.cache_batch, spp, args.ref_spp),
         "spp": spp,
         "plain": plain,
         "cached": cached,
@@ -330,6 +331,7 @@ def main():
     ap.add_argument("--cache-log2", type=int, default=15, help="cache workload: log2 of the hash-table entries per level")
     ap.add_argument("--cache-train-shift", type=int, default=7,
                     help="cache workload: one camera path in 2^shift trains")
+    ap.add_argument("--cache-lr", type=float, default=1e-2, help="cache workload: Adam learning rate")
     ap.add_argument("--cache-spp", type=int, default=64,
                     help="spp of each side of the cache-vs-plain comparison in the default line")
     ap.add_argument("--no-cache-compare", action="store_true",
