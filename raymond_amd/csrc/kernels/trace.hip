@@ -45,6 +45,7 @@ __device__ inline uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
 // flat access).
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 typedef __attribute__((address_space(1))) unsigned long long global_u64;
+typedef __attribute__((address_space(3))) float lds_f32;
 
 __device__ inline unsigned long long pack_entry(uint2 v) { return ((unsigned long long)v.y << 32) | v.x; }
 __device__ inline uint2 unpack_entry(unsigned long long w) { return make_uint2((uint32_t)w, (uint32_t)(w >> 32)); }
@@ -53,6 +54,25 @@ struct StackRef {
     lds_u64* lds;       // column base: lds[sp * 64]
     global_u64* spill;  // spill[(sp - kLdsStack) * stride]
     uint32_t stride;
+    // Two-level trees: the world ray's origin and direction, world[k * 64].
+    // It lives in LDS beside the stack rows rather than in six registers
+    // that every step would carry, since only entering and leaving an
+    // instance read it: the closest-hit kernel then fits 5 waves per SIMD
+    // without the 36 bytes per lane of spills it had, which measured 2346-2351
+    // against 2285 Mrays/s on the 1080p instanced field.
+    lds_f32* world;
+    __device__ void put_world(vec3 o, vec3 d) const {
+        world[0] = o.x;
+        world[kTraceBlock] = o.y;
+        world[2 * kTraceBlock] = o.z;
+        world[3 * kTraceBlock] = d.x;
+        world[4 * kTraceBlock] = d.y;
+        world[5 * kTraceBlock] = d.z;
+    }
+    __device__ vec3 world_o() const { return vec3(world[0], world[kTraceBlock], world[2 * kTraceBlock]); }
+    __device__ vec3 world_d() const {
+        return vec3(world[3 * kTraceBlock], world[4 * kTraceBlock], world[5 * kTraceBlock]);
+    }
     __device__ void store(int sp, uint2 v) const {
         if (sp < kLdsStack) lds[sp * kTraceBlock] = pack_entry(v);
         else spill[(size_t)(sp - kLdsStack) * stride] = pack_entry(v);
@@ -98,8 +118,7 @@ struct RayState {
     uint2 tg2;  // a second postponed leaf set (kLeafSlots == 2)
     int sp;
     uint32_t n_nodes, n_tris;
-    // two-level trees: the world ray, the instance entered, in the top level
-    vec3 wo, wd;
+    // two-level trees: the instance entered, in the top level
     uint32_t inst;
     bool top;
     int marker_sp;  // stack index of the instance's marker entry (-1 in the top level)
@@ -113,8 +132,6 @@ struct RayState {
 
     __device__ void init(uint32_t root, vec3 org, vec3 d, float t0, float t1) {
         set_ray(org, d);
-        wo = org;
-        wd = d;
         inst = 0;
         top = true;
         marker_sp = -1;
@@ -162,7 +179,7 @@ struct RayState {
             ti.m[4 * j + 2] = __uint_as_float(rows[j].z);
             ti.m[4 * j + 3] = __uint_as_float(rows[j].w);
         }
-        set_ray(tlas_point(ti, wo), tlas_vector(ti, wd));
+        set_ray(tlas_point(ti, stk.world_o()), tlas_vector(ti, stk.world_d()));
         inst = r3.y;
         top = false;
         cur = make_uint2(r3.x, (1u << (24 + oct)) | 1u);
@@ -214,11 +231,13 @@ struct RayState {
             const uint2 e = stk.load(sp);
             if (TWO && (e.y & kMarker2l)) {
                 // back in the world: the set's other instances, then the
-                // top-level groups below the marker
-                set_ray(wo, wd);
+                // top-level groups below the marker (entering the next
+                // instance maps the ray again, so the world ray is needed only
+                // after the set's last one)
+                tg = make_uint2(e.x, e.y & ~kMarker2l);
+                if (tg.y == 0u) set_ray(stk.world_o(), stk.world_d());
                 top = true;
                 marker_sp = -1;
-                tg = make_uint2(e.x, e.y & ~kMarker2l);
                 cur = make_uint2(0u, 0u);
                 return false;
             }
@@ -359,9 +378,10 @@ struct RayState {
 #define RM_TRACE_MIN_WAVES 1
 #endif
 constexpr int kTraceMinWaves = RM_TRACE_MIN_WAVES;
-// The two-level kernels take 103 and 101 VGPRs left free (4 waves); held to
-// 5 waves they take 96 with 24 and 20 bytes of spills per lane, which
-// measured 2331 against 2295 Mrays/s on the 1080p instanced field.
+// The two-level kernels take 98 and 95 VGPRs left free (4 and 5 waves per
+// SIMD); held to 5 waves both fit without spills.  With the world ray still
+// in registers they took 103 and 101 free, and 5 waves with spills measured
+// 2331 against 2295 Mrays/s for 4 waves on the 1080p instanced field.
 #ifndef RM_TWO_LEVEL_MIN_WAVES
 #define RM_TWO_LEVEL_MIN_WAVES 5
 #endif
@@ -460,9 +480,11 @@ struct RayPool {
 template <bool STATS, bool TWO>
 __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWaves) void trace_closest_kernel(ClosestArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
+    __shared__ float world[TWO ? 6 * kTraceBlock : 1];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock};
+    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock,
+                 (lds_f32*)(world + (TWO ? lane : 0u))};
     const uint32_t* count_ptr = a.count_ptr;
     uint32_t* fetch = a.fetch;
     if (a.iter_dev) {
@@ -504,6 +526,7 @@ __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWav
                 tmax = depth == 0 ? far_clip : (depth == kVoidDepth ? 0.0f : kInf);
             }
             s.init(root, o, d, tmin, tmax);
+            if (TWO) stk.put_world(o, d);
             active = true;
         }
         if (!__any(active)) break;
@@ -609,9 +632,11 @@ __global__ void regenerate_kernel(RegenArgs a) {
 template <bool STATS, bool TWO>
 __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWaves) void trace_any_kernel(AnyArgs a) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
+    __shared__ float world[TWO ? 6 * kTraceBlock : 1];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock};
+    StackRef stk{(lds_u64*)(lds + lane), (global_u64*)(a.spill + gtid), gridDim.x * kTraceBlock,
+                 (lds_f32*)(world + (TWO ? lane : 0u))};
     const uint32_t* count_ptr = a.count_ptr;
     uint32_t* fetch = a.fetch;
     if (a.iter_dev) {
@@ -638,6 +663,7 @@ __global__ __launch_bounds__(kTraceBlock, TWO ? kTwoLevelMinWaves : kTraceMinWav
             uint32_t root = TWO ? a.root2l : 0u;
             if (a.batch.jobs) root = a.batch.jobs[job_of_pixel(a.batch.jobs, a.batch.num_jobs, q.pixel[i])].root;
             s.init(root, o, d, tmin, tmax);
+            if (TWO) stk.put_world(o, d);
             active = true;
         }
         if (!__any(active)) break;
