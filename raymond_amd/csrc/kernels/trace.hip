@@ -102,6 +102,15 @@ constexpr int kLeafSlots = RM_LEAF_SLOTS;
 #define RM_HELD_SHARE 6
 #endif
 constexpr int kHeldShare = RM_HELD_SHARE;
+// The any-hit kernel, whose rays may end at their first hit, starts its
+// triangle phases sooner: at 1/3 with 3 triangles per phase, its time on
+// the sphere field measured 0.341-0.343 ms per spp against 0.353-0.354 at
+// the closest-hit kernel's 1/6 and 2 (1/4 and 3: 0.342-0.344; 1/4 and 4:
+// 0.342-0.344, slower on the instanced field; 1/10 and 2: 0.363).
+#ifndef RM_HELD_SHARE_ANY
+#define RM_HELD_SHARE_ANY 3
+#endif
+constexpr int kHeldShareAny = RM_HELD_SHARE_ANY;
 
 // Traversal state of one lane's ray.  It persists across steps, so a lane
 // whose ray has finished takes a new one while the rest of the wave goes on.
@@ -211,7 +220,8 @@ struct RayState {
         bool can_node = !to_enter && node_work && (kTwoSlots ? tg2.y == 0u : !has_tris);
         if (TWO && kTwoSlots && has_tris && !(cur.y & 0xff000000u) && sp - 1 == marker_sp) can_node = false;
         const unsigned long long held = __ballot((has_tris || to_enter) && !can_node);
-        if (__popcll(held) * kHeldShare >= __popcll(__ballot(true)) || __ballot(can_node) == 0ull) {
+        if (__popcll(held) * (ANY ? kHeldShareAny : kHeldShare) >= __popcll(__ballot(true)) ||
+            __ballot(can_node) == 0ull) {
             if (to_enter) {
                 enter(insts, stk);
                 return false;
@@ -340,9 +350,14 @@ struct RayState {
 #define RM_TRIS_PER_PHASE 2
 #endif
     static constexpr int kTrisPerPhase = RM_TRIS_PER_PHASE;
+// (any hit: 1 per phase 0.380 ms per spp against 0.354 for 2, 3 0.349-0.350)
+#ifndef RM_TRIS_PER_PHASE_ANY
+#define RM_TRIS_PER_PHASE_ANY 3
+#endif
+    static constexpr int kTrisPerPhaseAny = RM_TRIS_PER_PHASE_ANY;
     template <bool ANY, bool STATS, bool TWO>
     __device__ bool test_tris(const BvhTri* __restrict__ tris) {
-        for (int k = 0; k < kTrisPerPhase && tg.y; k++) {
+        for (int k = 0; k < (ANY ? kTrisPerPhaseAny : kTrisPerPhase) && tg.y; k++) {
             const int b = __builtin_ctz(tg.y);
             tg.y &= tg.y - 1u;
             const uint4* tp = reinterpret_cast<const uint4*>(tris + tg.x + b);
