@@ -345,7 +345,9 @@ struct RayState {
     // left for node steps while others still have some.  On the 5.2M-triangle
     // sphere field (Mrays/s), one leaf slot: 1 per phase 2841, 2 per phase
     // 2909, 3 2891, 4 2870, emptying every lane's list per phase slower than
-    // 1; two slots: 1 2899, 2 3033-3071, 3 3052.
+    // 1; two slots: 1 2899, 2 3033-3071, 3 3052.  Three per phase with the
+    // phase started at 1/4, 1/5 or 1/8 of the lanes held measured 3050-3084
+    // against 3104-3109 for two at 1/6 (the same box).
 #ifndef RM_TRIS_PER_PHASE
 #define RM_TRIS_PER_PHASE 2
 #endif
